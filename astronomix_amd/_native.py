@@ -1,0 +1,186 @@
+"""ctypes binding of ``libamx.so`` (the C ABI declared in ``include/amx.h``).
+
+The library is built in-tree by ``__graft_entry__.build()`` /
+``astronomix_amd.build``. There is no CPU fallback: if the library is absent,
+or no GPU is visible, every entry point raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libamx.so")
+
+AMX_OK = 0
+AMX_HLL = 0
+AMX_HLLC = 1
+
+
+class AmxFvConfig(ctypes.Structure):
+    _fields_ = [
+        ("dim", ctypes.c_int),
+        ("nx", ctypes.c_int),
+        ("ny", ctypes.c_int),
+        ("nz", ctypes.c_int),
+        ("mhd", ctypes.c_int),
+        ("riemann", ctypes.c_int),
+        ("magnetic_integrator", ctypes.c_int),
+        ("max_fixed_point_iterations", ctypes.c_int),
+        ("gamma", ctypes.c_float),
+        ("C_cfl", ctypes.c_float),
+        ("fixed_point_tol", ctypes.c_float),
+        ("grid_spacing", ctypes.c_double),
+    ]
+
+
+class AmxFvRun(ctypes.Structure):
+    _fields_ = [
+        ("t_end", ctypes.c_float),
+        ("fixed_timestep", ctypes.c_int),
+        ("num_timesteps", ctypes.c_int),
+        ("exact_end_time", ctypes.c_int),
+        ("max_steps", ctypes.c_long),
+    ]
+
+
+class AmxFvStats(ctypes.Structure):
+    _fields_ = [
+        ("steps", ctypes.c_long),
+        ("t", ctypes.c_double),
+        ("fixed_point_iterations", ctypes.c_long),
+        ("max_fixed_point_iterations", ctypes.c_int),
+    ]
+
+
+class AmxFvTiming(ctypes.Structure):
+    _fields_ = [
+        ("stage_ms", ctypes.c_double),
+        ("stage_launches", ctypes.c_long),
+        ("mag_ms", ctypes.c_double),
+        ("mag_launches", ctypes.c_long),
+    ]
+
+
+# Every symbol include/amx.h declares, with its ctypes signature.
+_P = ctypes.c_void_p
+SIGNATURES = {
+    "amx_fv_create": (ctypes.c_int, [ctypes.POINTER(AmxFvConfig), ctypes.POINTER(_P)]),
+    "amx_fv_destroy": (ctypes.c_int, [_P]),
+    "amx_fv_workspace_bytes": (ctypes.c_int, [ctypes.POINTER(AmxFvConfig), ctypes.POINTER(ctypes.c_size_t)]),
+    "amx_fv_cfl_dt": (ctypes.c_int, [_P, _P, _P, _P]),
+    "amx_fv_evolve_state": (ctypes.c_int, [_P, _P, _P, ctypes.POINTER(ctypes.c_int), _P]),
+    "amx_fv_time_integration": (ctypes.c_int, [_P, _P, ctypes.POINTER(AmxFvRun), ctypes.POINTER(AmxFvStats), _P]),
+    "amx_fv_set_timing": (ctypes.c_int, [_P, ctypes.c_int]),
+    "amx_fv_get_timing": (ctypes.c_int, [_P, ctypes.POINTER(AmxFvTiming)]),
+    "amx_status_string": (ctypes.c_char_p, [ctypes.c_int]),
+    "amx_last_error": (ctypes.c_char_p, []),
+}
+
+_lib = None
+_lock = threading.Lock()
+
+
+class AmxError(RuntimeError):
+    pass
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libamx.so (import torch first so both share one HIP runtime)."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(path):
+                raise AmxError(
+                    f"{path} is not built: the HIP extension is required (run __graft_entry__.build())")
+            import torch  # noqa: F401  -- load torch's HIP runtime first
+            lib = ctypes.CDLL(path)
+            for name, (res, args) in SIGNATURES.items():
+                fn = getattr(lib, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = lib
+    return _lib
+
+
+def check(status: int, what: str):
+    if status != AMX_OK:
+        lib = load_library()
+        msg = lib.amx_last_error().decode(errors="replace")
+        raise AmxError(f"{what} failed: {lib.amx_status_string(status).decode()} ({msg})")
+
+
+def require_gpu():
+    import torch
+    if not torch.cuda.is_available():
+        raise AmxError("no GPU visible: the astronomix_amd step runs only on the HIP device")
+
+
+class FvContext:
+    """Owns one amx_fv_ctx (device workspace for one grid/configuration)."""
+
+    def __init__(self, cfg: AmxFvConfig):
+        require_gpu()
+        self.lib = load_library()
+        self.cfg = cfg
+        h = ctypes.c_void_p()
+        check(self.lib.amx_fv_create(ctypes.byref(cfg), ctypes.byref(h)), "amx_fv_create")
+        self.handle = h
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.amx_fv_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _stream(stream):
+        import torch
+        s = stream if stream is not None else torch.cuda.current_stream()
+        return ctypes.c_void_p(s.cuda_stream)
+
+    def cfl_dt(self, state, out, stream=None):
+        check(self.lib.amx_fv_cfl_dt(self.handle, ctypes.c_void_p(state.data_ptr()),
+                                     ctypes.c_void_p(out.data_ptr()), self._stream(stream)), "amx_fv_cfl_dt")
+
+    def evolve_state(self, state, dt, stream=None) -> int:
+        k = ctypes.c_int(0)
+        check(self.lib.amx_fv_evolve_state(self.handle, ctypes.c_void_p(state.data_ptr()),
+                                           ctypes.c_void_p(dt.data_ptr()), ctypes.byref(k),
+                                           self._stream(stream)), "amx_fv_evolve_state")
+        return int(k.value)
+
+    def set_timing(self, enable: bool):
+        check(self.lib.amx_fv_set_timing(self.handle, int(bool(enable))), "amx_fv_set_timing")
+
+    def get_timing(self) -> AmxFvTiming:
+        t = AmxFvTiming()
+        check(self.lib.amx_fv_get_timing(self.handle, ctypes.byref(t)), "amx_fv_get_timing")
+        return t
+
+    def time_integration(self, state, run: AmxFvRun, stream=None) -> AmxFvStats:
+        st = AmxFvStats()
+        check(self.lib.amx_fv_time_integration(self.handle, ctypes.c_void_p(state.data_ptr()), ctypes.byref(run),
+                                               ctypes.byref(st), self._stream(stream)), "amx_fv_time_integration")
+        return st
+
+
+_ctx_cache = {}
+
+
+def fv_context(cfg: AmxFvConfig) -> FvContext:
+    """Cached context per (device, configuration)."""
+    import torch
+    key = (torch.cuda.current_device(),) + tuple(getattr(cfg, f) for f, _ in AmxFvConfig._fields_)
+    ctx = _ctx_cache.get(key)
+    if ctx is None:
+        ctx = FvContext(cfg)
+        _ctx_cache[key] = ctx
+    return ctx

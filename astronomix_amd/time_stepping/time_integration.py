@@ -1,0 +1,83 @@
+"""``time_integration`` -- the public entry point of the reference
+(``astronomix/time_stepping/time_integration.py:61-236``) on the HIP path.
+
+Accepts the reference's arguments and returns the final primitive state in
+the same layout. The state may be a numpy array (copied to the GPU and back)
+or a float32 torch tensor on the GPU (returned as a new tensor, or updated in
+place when ``config.donate_state``). The step loop runs on the device: dt is
+reduced, clamped to ``t_end - t`` and consumed on the GPU
+(time_integration.py:629-801).
+"""
+
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from astronomix_amd import _native
+from astronomix_amd._finite_volume import native_config, require_supported
+from astronomix_amd.option_classes.simulation_config import FINITE_DIFFERENCE
+
+
+def time_integration(primitive_state, config, params, registered_variables, snapshot_callable=None,
+                     sharding=None):
+    import torch
+
+    if config.return_snapshots or config.activate_snapshot_callback:
+        raise NotImplementedError("snapshot output (SnapshotData / snapshot callbacks) is not implemented")
+    if sharding is not None:
+        raise NotImplementedError("sharded states are not supported: one GPU per simulation")
+    if config.solver_mode == FINITE_DIFFERENCE:
+        from astronomix_amd._finite_difference import time_integration_fd
+        return time_integration_fd(primitive_state, config, params, registered_variables)
+    require_supported(config, registered_variables)
+
+    struct = None
+    if config.state_struct:
+        struct = primitive_state
+        primitive_state = struct.primitive_state
+
+    _native.require_gpu()
+    host_numpy = not isinstance(primitive_state, torch.Tensor)
+    if host_numpy:
+        arr = np.ascontiguousarray(np.asarray(primitive_state, dtype=np.float32))
+        state = torch.from_numpy(arr).to("cuda")
+    else:
+        if primitive_state.device.type != "cuda" or primitive_state.dtype != torch.float32:
+            raise TypeError("a tensor state must be float32 on the GPU")
+        state = primitive_state if (config.donate_state and primitive_state.is_contiguous()) \
+            else primitive_state.contiguous().clone()
+
+    ctx = _native.fv_context(native_config(state.shape, config, params.gamma, params.C_cfl))
+    run = _native.AmxFvRun(t_end=float(params.t_end), fixed_timestep=int(bool(config.fixed_timestep)),
+                           num_timesteps=int(config.num_timesteps), exact_end_time=int(bool(config.exact_end_time)),
+                           max_steps=-1)
+    if config.memory_analysis:
+        import ctypes
+        nbytes = ctypes.c_size_t(0)
+        ctx.lib.amx_fv_workspace_bytes(ctypes.byref(ctx.cfg), ctypes.byref(nbytes))
+        print("=== device memory PER DEVICE ===")
+        print(f"State size: {state.numel() * 4 / 1024**2:.2f} MB")
+        print(f"Workspace size: {nbytes.value / 1024**2:.2f} MB")
+    t0 = time.perf_counter()
+    if config.print_elapsed_time:
+        print("Starting simulation...")
+    stats = ctx.time_integration(state, run)
+    if config.print_elapsed_time:
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        print("Simulation finished!")
+        print(f"Time elapsed: {el:.2f} seconds")
+        print(f"Number of iterations: {stats.steps}")
+    if config.runtime_debugging:
+        gas = state[: registered_variables.pressure_index + 1]
+        if not torch.isfinite(state).all():
+            raise FloatingPointError("non-finite values in the final state")
+        if (gas[registered_variables.pressure_index] < 0).any() or (gas[registered_variables.density_index] < 0).any():
+            raise FloatingPointError("negative density or pressure in the final state")
+    out = state.cpu().numpy() if host_numpy else state
+    if struct is not None:
+        return struct._replace(primitive_state=out)
+    return out
+

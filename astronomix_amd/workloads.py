@@ -1,0 +1,134 @@
+"""Synthetic initial states of the benchmark and parity workloads.
+
+Deterministic numpy builders in the reference's state layout (float32 unless
+``dtype`` says otherwise); seeds only where a field is random.
+
+* ``orszag_tang_2d``  -- 2D Orszag-Tang vortex on [0,1]^2 (cell centres):
+  rho = 25/(36 pi), p = 5/(12 pi), v = (-sin 2pi y, sin 2pi x),
+  B = B0 (-sin 2pi y, sin 4pi x), B0 = 1/sqrt(4 pi).
+* ``blast_3d``        -- 3D MHD blast on [0,1]^3: rho = 1, p = p_in inside
+  r < r0 of the centre else p_out, v = 0, uniform B = (1,1,0)/sqrt(2).
+* ``kelvin_helmholtz_2d`` -- 2D hydro shear layer on [0,1]^2: rho = 2 for
+  |y-0.5| < 0.25 else 1, vx = +-0.5, p = 2.5, vy = 0.01 sin(4 pi x) plus
+  seeded per-cell N(0, 1e-3).
+* ``random_smooth``   -- smooth periodic random state for kernel parity tests.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from astronomix_amd.option_classes.simulation_config import (
+    ALL_PERIODIC,
+    HLL,
+    HLLC,
+    SimulationConfig,
+    finalize_config,
+)
+from astronomix_amd.option_classes.simulation_params import SimulationParams
+
+
+def cell_centres(n: int, box: float = 1.0, dtype=np.float64) -> np.ndarray:
+    h = box / n
+    return ((np.arange(n, dtype=np.float64) + 0.5) * h).astype(dtype)
+
+
+def orszag_tang_2d(n: int = 256, dtype=np.float32) -> np.ndarray:
+    x = cell_centres(n)
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    B0 = 1.0 / np.sqrt(4.0 * np.pi)
+    s = np.zeros((7, n, n), dtype=np.float64)
+    s[0] = 25.0 / (36.0 * np.pi)
+    s[1] = -np.sin(2 * np.pi * Y)
+    s[2] = np.sin(2 * np.pi * X)
+    s[3] = 5.0 / (12.0 * np.pi)
+    s[4] = -B0 * np.sin(2 * np.pi * Y)
+    s[5] = B0 * np.sin(4 * np.pi * X)
+    return s.astype(dtype)
+
+
+def blast_3d(n: int = 256, p_in: float = 10.0, p_out: float = 0.1, r0: float = 0.1,
+             B=(1.0 / np.sqrt(2.0), 1.0 / np.sqrt(2.0), 0.0), dtype=np.float32) -> np.ndarray:
+    x = cell_centres(n)
+    r2 = ((x[:, None, None] - 0.5) ** 2 + (x[None, :, None] - 0.5) ** 2 + (x[None, None, :] - 0.5) ** 2)
+    s = np.zeros((8, n, n, n), dtype=dtype)
+    s[0] = 1.0
+    s[4] = np.where(r2 < r0 * r0, p_in, p_out)
+    s[5] = B[0]
+    s[6] = B[1]
+    s[7] = B[2]
+    return s
+
+
+def kelvin_helmholtz_2d(n: int = 512, seed: int = 0, dtype=np.float32) -> np.ndarray:
+    x = cell_centres(n)
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    rng = np.random.default_rng(seed)
+    inner = np.abs(Y - 0.5) < 0.25
+    s = np.zeros((4, n, n), dtype=np.float64)
+    s[0] = np.where(inner, 2.0, 1.0)
+    s[1] = np.where(inner, 0.5, -0.5)
+    s[2] = 0.01 * np.sin(4 * np.pi * X) + rng.normal(0.0, 1e-3, size=(n, n))
+    s[3] = 2.5
+    return s.astype(dtype)
+
+
+def random_smooth(shape, mhd: bool, seed: int = 0, amp: float = 0.2, dtype=np.float32) -> np.ndarray:
+    """Positive density/pressure and O(1) velocities/fields made of a few
+    periodic Fourier modes (so minmod sees both smooth regions and extrema)."""
+    rng = np.random.default_rng(seed)
+    dim = len(shape)
+    grids = np.meshgrid(*[cell_centres(n) for n in shape], indexing="ij")
+    nv = dim + 2 + (3 if mhd else 0)
+
+    def field():
+        f = np.zeros(shape)
+        for _ in range(3):
+            k = rng.integers(1, 4, size=dim)
+            ph = rng.uniform(0, 2 * np.pi)
+            f += rng.normal() * np.sin(2 * np.pi * sum(k[i] * grids[i] for i in range(dim)) + ph)
+        return f
+
+    s = np.zeros((nv,) + tuple(shape))
+    s[0] = 1.0 + amp * np.tanh(field())
+    for i in range(1, dim + 1):
+        s[i] = 0.5 * field()
+    s[dim + 1] = 1.0 + amp * np.tanh(field())
+    if mhd:
+        # divergence-free B from a vector potential (central differences keep
+        # div B = 0 only approximately; the fixed point preserves whatever div B is)
+        for i in range(3):
+            s[dim + 2 + i] = 0.3 * field()
+    return s.astype(dtype)
+
+
+def fv_config(dim: int, n: int, *, mhd: bool = True, riemann: int = HLL, box_size: float = 1.0,
+              **kw) -> SimulationConfig:
+    cfg = SimulationConfig(dimensionality=dim, mhd=mhd, riemann_solver=riemann, box_size=box_size,
+                           num_cells=n, boundary_settings=ALL_PERIODIC, **kw)
+    shape = (dim + 2 + (3 if mhd else 0),) + (n,) * dim
+    return finalize_config(cfg, shape)
+
+
+WORKLOADS = {
+    # name: (builder, config kwargs, params kwargs)
+    "orszag_tang_2d": dict(dim=2, mhd=True, riemann=HLL, C_cfl=0.4, t_end=0.5, gamma=5.0 / 3.0),
+    "mhd_blast_3d": dict(dim=3, mhd=True, riemann=HLL, C_cfl=0.3, t_end=1e9, gamma=5.0 / 3.0),
+    "kelvin_helmholtz_2d": dict(dim=2, mhd=False, riemann=HLLC, C_cfl=0.4, t_end=1.0, gamma=5.0 / 3.0),
+}
+
+
+def workload(name: str, n: int):
+    """(state, config, params) of a named workload at resolution n."""
+    spec = WORKLOADS[name]
+    if name == "orszag_tang_2d":
+        state = orszag_tang_2d(n)
+    elif name == "mhd_blast_3d":
+        state = blast_3d(n)
+    elif name == "kelvin_helmholtz_2d":
+        state = kelvin_helmholtz_2d(n)
+    else:
+        raise KeyError(name)
+    cfg = fv_config(spec["dim"], n, mhd=spec["mhd"], riemann=spec["riemann"])
+    params = SimulationParams(C_cfl=spec["C_cfl"], t_end=spec["t_end"], gamma=spec["gamma"])
+    return state, cfg, params

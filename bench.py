@@ -1,0 +1,170 @@
+"""Benchmark: 3D FV MHD step, 256^3 blast (HLL, minmod, SSP-RK2 half steps
+around the implicit-midpoint magnetic sub-step, CFL 0.3, periodic, float32)
+on one MI355X.
+
+A "step" is one full _evolve_state_fv step including its CFL dt reduction.
+The state stays resident in HBM for the whole timed region. Prints one JSON
+line (rank 0 only; this build runs on one GPU).
+
+    python bench.py [--steps 200] [--warmup 10] [--n 256] [--no-cpu-baseline]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+if HERE not in sys.path:
+    sys.path.insert(0, HERE)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E datasheet peak
+# Algorithmic bytes per cell of one launch (float32 fields read + written once):
+#   k_mag   : reads B_k, v_k, B0, v0, rho (13 fields), writes B_k+1, v_k+1 (6)  -> 76 B
+#   k_stage : mean over the four launches of a step
+#             A1 (R5 W5) + A2 (R10 W5) + A1' (R8 W5) + A2' (R16 W8) = 62 fields -> 62 B
+MAG_BYTES_PER_CELL = 76.0
+STAGE_BYTES_PER_CELL = 62.0
+# Whole step: 4 stage passes (62 fields) + magnetic init (13) + 19 per iteration
+STEP_FIELDS_BASE = 62 + 13
+
+
+def cpu_baseline(n: int, steps: int):
+    """Time the CPU restatement (oracle/fv_oracle.cpp, float32, OpenMP on all
+    host cores of this process) on a bounded sample of the same workload."""
+    import numpy as np
+    from oracle import fv
+    from astronomix_amd import workloads as W
+    fv.build()
+    cores = len(os.sched_getaffinity(0))
+    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
+    state = W.blast_3d(n)
+    oc = fv.make_config(state.shape, 3, C_cfl=0.3, t_end=1e9)
+    t0 = time.perf_counter()
+    _, done, _, _ = fv.time_integration(state, oc, max_steps=steps)
+    el = time.perf_counter() - t0
+    return {
+        "value": round(done * n ** 3 / el / 1e6, 4),
+        "unit": "Mzone-updates/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"mhd_blast_3d {n}^3, {done} steps from the initial state, float32 C++/OpenMP "
+                  f"restatement (repo CPU restatement, not reference JAX); cpu={platform.processor() or platform.machine()}",
+        "seconds": round(el, 3),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from astronomix_amd import _native
+    from astronomix_amd import workloads as W
+    from astronomix_amd._finite_volume import native_config
+
+    n = args.n
+    state_np, cfg, params = W.workload("mhd_blast_3d", n)
+    state = torch.from_numpy(state_np).cuda()
+    ctx = _native.fv_context(native_config(state.shape, cfg, params.gamma, params.C_cfl))
+    run_w = _native.AmxFvRun(t_end=float(params.t_end), fixed_timestep=0, num_timesteps=0, exact_end_time=1,
+                             max_steps=args.warmup)
+    if args.warmup > 0:
+        ctx.time_integration(state, run_w)
+    torch.cuda.synchronize()
+    ctx.set_timing(True)
+    run = _native.AmxFvRun(t_end=float(params.t_end), fixed_timestep=0, num_timesteps=0, exact_end_time=1,
+                           max_steps=args.steps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = ctx.time_integration(state, run)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    tm = ctx.get_timing()
+    ctx.set_timing(False)
+    if not bool(torch.isfinite(state).all()):
+        print("non-finite state after the timed steps", file=sys.stderr)
+        sys.exit(1)
+
+    cells = n ** 3
+    steps = st.steps
+    value = steps * cells / el / 1e6
+    kbar = st.fixed_point_iterations / max(steps, 1)
+    step_bytes = (STEP_FIELDS_BASE + 19.0 * kbar) * 4.0
+    mag_avg_ms = tm.mag_ms / max(tm.mag_launches, 1)
+    stage_avg_ms = tm.stage_ms / max(tm.stage_launches, 1)
+    if tm.mag_ms >= tm.stage_ms:
+        dom, avg_ms, per_cell = "k_mag", mag_avg_ms, MAG_BYTES_PER_CELL
+    else:
+        dom, avg_ms, per_cell = "k_stage", stage_avg_ms, STAGE_BYTES_PER_CELL
+    achieved = per_cell * cells / (avg_ms * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(HERE, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        try:
+            traffic = json.load(open(tpath)).get(f"{dom}_{n}")
+        except Exception:
+            traffic = None
+    out = {
+        "metric": "Mzone-updates/s, 3D FV MHD 256^3 fp32 (% HBM roofline)",
+        "value": round(value, 3),
+        "unit": "Mzone-updates/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(el / max(steps, 1) * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"mhd_blast_3d_{n}: FV MHD, HLL + minmod, SSP-RK2 half steps around the implicit-midpoint "
+                        f"magnetic fixed point, CFL 0.3, periodic, rho=1, p=10 (r<0.1) / 0.1, v=0, B=(1,1,0)/sqrt2",
+            "grid": [n, n, n],
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom,
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "bytes_per_cell_per_launch": per_cell,
+            "avg_launch_ms": round(avg_ms, 5),
+        },
+        "step_model": {
+            "fixed_point_iterations_per_step": round(kbar, 3),
+            "bytes_per_zone_update": round(step_bytes, 1),
+            "effective_GBs": round(step_bytes * cells * steps / el / 1e9, 1),
+            "frac_of_peak": round(step_bytes * cells * steps / el / 1e9 / HBM_PEAK_GBS, 4),
+            "k_mag_avg_ms": round(mag_avg_ms, 5),
+            "k_mag_launches": tm.mag_launches,
+            "k_stage_avg_ms": round(stage_avg_ms, 5),
+            "k_stage_launches": tm.stage_launches,
+        },
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(n, args.cpu_steps)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,132 @@
+/*
+ * amx.h -- C ABI of the MI355X-native per-step MHD update (libamx.so).
+ *
+ * Plain C: device pointers, sizes, scalar structs and an opaque context. No
+ * torch or C++ types cross this boundary. Streams are hipStream_t passed as
+ * void* (NULL = the default stream). Every entry point returns an amx_status;
+ * no exception crosses the ABI.
+ *
+ * State layout is the reference's (astronomix/variable_registry/
+ * registered_variables.py:116-238): a contiguous float32 array
+ * (num_vars, nx, ny[, nz]), row-major, with
+ *   2D hydro [rho, vx, vy, p]            2D MHD [rho, vx, vy, p, Bx, By, Bz]
+ *   3D hydro [rho, vx, vy, vz, p]        3D MHD [rho, vx, vy, vz, p, Bx, By, Bz]
+ * Boundaries are periodic on every axis.
+ *
+ * The reference has no FFI layer of its own; each entry point below replaces
+ * one JAX operator of the reference's finite-volume path, cited per function.
+ */
+#ifndef AMX_H
+#define AMX_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum amx_status {
+  AMX_OK = 0,
+  AMX_ERR_INVALID = 1,       /* bad argument or unsupported configuration */
+  AMX_ERR_HIP = 2,           /* a HIP runtime call failed */
+  AMX_ERR_NOMEM = 3,         /* device allocation failed */
+  AMX_ERR_UNSUPPORTED = 4    /* option outside the implemented path */
+} amx_status;
+
+/* Riemann solvers (simulation_config.py:41-46 values). */
+#define AMX_HLL 0
+#define AMX_HLLC 1
+/* Magnetic sub-step integrators (simulation_config.py:88-89 values). */
+#define AMX_IMPLICIT_MIDPOINT 0
+#define AMX_IMPLICIT_EULER 1
+
+/* Static description of one finite-volume simulation; the fields correspond to
+ * SimulationConfig / SimulationParams (option_classes/simulation_config.py:160-342,
+ * option_classes/simulation_params.py:11-47). */
+typedef struct amx_fv_config {
+  int dim;                        /* 2 or 3 (config.dimensionality) */
+  int nx, ny, nz;                 /* cells per axis; nz = 1 in 2D */
+  int mhd;                        /* config.mhd */
+  int riemann;                    /* AMX_HLL / AMX_HLLC (config.riemann_solver) */
+  int magnetic_integrator;        /* config.fv_magnetic_integrator */
+  int max_fixed_point_iterations; /* 1000 in the reference (_magnetic_field_update.py:141) */
+  float gamma;                    /* params.gamma */
+  float C_cfl;                    /* params.C_cfl */
+  float fixed_point_tol;          /* 1e-5 for SINGLE_PRECISION (_magnetic_field_update.py:134-139) */
+  double grid_spacing;            /* config.grid_spacing = box_size / num_cells */
+} amx_fv_config;
+
+/* Loop controls of time_integration (time_integration.py:629-801). */
+typedef struct amx_fv_run {
+  float t_end;                    /* params.t_end */
+  int fixed_timestep;             /* config.fixed_timestep */
+  int num_timesteps;              /* config.num_timesteps */
+  int exact_end_time;             /* config.exact_end_time */
+  long max_steps;                 /* < 0: no cap (benchmarking uses a cap) */
+} amx_fv_run;
+
+typedef struct amx_fv_stats {
+  long steps;                     /* steps taken */
+  double t;                       /* final simulation time (float32 value) */
+  long fixed_point_iterations;    /* sum over steps of while-body iterations */
+  int max_fixed_point_iterations; /* largest per-step count */
+} amx_fv_stats;
+
+typedef struct amx_fv_ctx amx_fv_ctx;
+
+/* Creates a context on the current HIP device and allocates its workspace
+ * (one stage buffer of the gas rows and two magnetic iterate buffers).
+ * Replaces the jit/compile setup of time_integration (time_integration.py:108-140). */
+amx_status amx_fv_create(const amx_fv_config* cfg, amx_fv_ctx** out);
+
+/* Frees the context and its workspace. */
+amx_status amx_fv_destroy(amx_fv_ctx* ctx);
+
+/* Bytes of device workspace a context of this configuration allocates. */
+amx_status amx_fv_workspace_bytes(const amx_fv_config* cfg, size_t* bytes);
+
+/* CFL time step of the unsplit scheme, written to d_dt (one device float).
+ * Replaces _cfl_time_step, UNSPLIT branch
+ * (_finite_volume/_timestep_estimation/_timestep_estimator.py:94-128). */
+amx_status amx_fv_cfl_dt(amx_fv_ctx* ctx, const float* state, float* d_dt, void* stream);
+
+/* One full step in place on `state` with the step size read from the device
+ * float d_dt. MHD: S_A(dt/2) o S_B(dt) o S_A(dt/2) with the Pang-Wu fixed
+ * point; hydro: one SSP-RK2 step. Blocks the calling thread once per step to
+ * read the fixed-point convergence state. *iterations (optional) receives the
+ * number of fixed-point while-body iterations.
+ * Replaces _evolve_state_fv (_finite_volume/_state_evolution/evolve_state.py:496-600). */
+amx_status amx_fv_evolve_state(amx_fv_ctx* ctx, float* state, const float* d_dt, int* iterations,
+                               void* stream);
+
+/* Device-resident step loop to t_end (or num_timesteps fixed steps), in place.
+ * Replaces the loop of _time_integration (time_integration.py:629-801) with
+ * snapshots off. */
+amx_status amx_fv_time_integration(amx_fv_ctx* ctx, float* state, const amx_fv_run* run,
+                                   amx_fv_stats* stats, void* stream);
+
+/* Per-kernel-class device time of the launches made while timing is on,
+ * measured with HIP events on the launch stream. Fixed-point launches that
+ * exit early (already converged) are not counted. */
+typedef struct amx_fv_timing {
+  double stage_ms;       /* gas stage kernels (k_stage) */
+  long stage_launches;
+  double mag_ms;         /* magnetic fixed-point kernels (k_mag), executed ones */
+  long mag_launches;
+} amx_fv_timing;
+
+/* Enables (1) / disables (0) event timing on ctx and resets its counters.
+ * No counterpart in the reference (its timing is wall-clock around the jitted
+ * loop, time_integration.py:194-234); used by the benchmark harness. */
+amx_status amx_fv_set_timing(amx_fv_ctx* ctx, int enable);
+amx_status amx_fv_get_timing(amx_fv_ctx* ctx, amx_fv_timing* out);
+
+/* Static description of the last error of this thread. */
+const char* amx_status_string(amx_status s);
+const char* amx_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AMX_H */
