@@ -12,7 +12,7 @@ import os
 import threading
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libamx.so")
+LIB_PATH = os.environ.get("AMX_LIB") or os.path.join(_HERE, "libamx.so")
 
 AMX_OK = 0
 AMX_HLL = 0
@@ -75,6 +75,7 @@ SIGNATURES = {
     "amx_fv_time_integration": (ctypes.c_int, [_P, _P, ctypes.POINTER(AmxFvRun), ctypes.POINTER(AmxFvStats), _P]),
     "amx_fv_set_timing": (ctypes.c_int, [_P, ctypes.c_int]),
     "amx_fv_get_timing": (ctypes.c_int, [_P, ctypes.POINTER(AmxFvTiming)]),
+    "amx_copy_f32": (ctypes.c_int, [_P, _P, ctypes.c_size_t, _P]),
     "amx_status_string": (ctypes.c_char_p, [ctypes.c_int]),
     "amx_last_error": (ctypes.c_char_p, []),
 }
@@ -170,6 +171,16 @@ class FvContext:
         check(self.lib.amx_fv_time_integration(self.handle, ctypes.c_void_p(state.data_ptr()), ctypes.byref(run),
                                                ctypes.byref(st), self._stream(stream)), "amx_fv_time_integration")
         return st
+
+
+def copy_f32(dst, src, stream=None):
+    """dst[:] = src[:] on the device with the library's streaming copy kernel."""
+    import torch
+    lib = load_library()
+    assert dst.numel() == src.numel() and dst.dtype == src.dtype == torch.float32
+    s = stream if stream is not None else torch.cuda.current_stream()
+    check(lib.amx_copy_f32(ctypes.c_void_p(dst.data_ptr()), ctypes.c_void_p(src.data_ptr()), dst.numel(),
+                           ctypes.c_void_p(s.cuda_stream)), "amx_copy_f32")
 
 
 _ctx_cache = {}

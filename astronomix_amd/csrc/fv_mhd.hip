@@ -27,6 +27,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <cstdint>
 #include <new>
 #include <vector>
 
@@ -35,17 +36,27 @@
 namespace amx {
 
 constexpr int TB = 64;      // tile width along the contiguous axis (one wavefront)
-constexpr int MAXRES = 1030;  // residual slots (fixed-point cap + 2)
+#ifndef AMX_STAGE_WAVES
+#define AMX_STAGE_WAVES 5   // >= 5 waves/SIMD keeps two 9-wave stage blocks per CU
+#endif
+constexpr int NSLOT = 64;   // partial-max slots (one cache line each is not needed: 64 x 4 B)
 
 // Device-resident step control: step size, time, fixed-point bookkeeping and
-// the CFL maxima (as float bits; all values are >= 0 so unsigned max == float max).
+// the CFL maxima. Reductions go through NSLOT partial slots (block b adds into
+// slot b % NSLOT, one atomic per block) so that thousands of blocks do not
+// serialise on one address; the consumer reduces the slots. All reduced values
+// are >= 0 floats stored as bits, so unsigned max == float max (NaN, made
+// canonical 0x7fc00000, wins as in jnp.max).
 struct Ctl {
   float dt;
   float t;
-  int iters;
+  int iters;   // fixed-point while-body iterations of the current step
   int steps;
-  unsigned cfl[4];
-  unsigned res[MAXRES];
+  int done;    // fixed point finished (sticky until the next step)
+  int pad[3];
+  unsigned cfl[4];              // reduced CFL maxima per axis
+  unsigned cflp[3][NSLOT];      // CFL partial maxima
+  unsigned resp[3][NSLOT];      // fixed-point residual partials, rotating by iteration % 3
 };
 
 struct Phys {
@@ -64,12 +75,15 @@ __device__ __forceinline__ int wrapi(int i, int n) {
   return i;
 }
 
+// Hardware reciprocal / square root (v_rcp_f32, v_sqrt_f32: <= 1 ulp) for
+// data-dependent divides and roots in the stencil kernels.
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 // limiters.py:13-15: 0.5 (sgn a + sgn b) min(|a|,|b|) is the argument of
-// smaller magnitude when both have the same strict sign, and 0 otherwise.
-__device__ __forceinline__ float minmod(float a, float b) {
-  const bool same = (a > 0.f && b > 0.f) || (a < 0.f && b < 0.f);
-  return same ? (fabsf(a) < fabsf(b) ? a : b) : 0.f;
-}
+// smaller magnitude when both have the same strict sign, and 0 otherwise --
+// i.e. the median of {a, b, 0}: one v_med3_f32.
+__device__ __forceinline__ float minmod(float a, float b) { return __builtin_amdgcn_fmed3f(a, b, 0.f); }
 
 // limited gradient times dx/2 (limited_gradients.py:50-60, reconstruction.py:409-412)
 __device__ __forceinline__ float half_slope(float wm, float w0, float wp) {
@@ -97,7 +111,7 @@ __device__ __forceinline__ void cons(const float* w, float* u, const Phys& ph) {
 template <int DIM>
 __device__ __forceinline__ void prim(const float* u, float* w, const Phys& ph) {
   constexpr int P = DIM + 1;
-  const float ri = 1.0f / u[0];
+  const float ri = frcp(u[0]);
   w[0] = u[0];
 #pragma unroll
   for (int i = 1; i <= DIM; ++i) w[i] = u[i] * ri;
@@ -113,13 +127,13 @@ __device__ __forceinline__ void flux_hll(const float* wl, const float* wr, float
   constexpr int P = DIM + 1;
   const float rl = wl[0], ul = wl[AX], pl = wl[P];
   const float rr = wr[0], ur = wr[AX], pr = wr[P];
-  const float cl = sqrtf(ph.gamma * pl / rl);
-  const float cr = sqrtf(ph.gamma * pr / rr);
+  const float cl = fsqrt(ph.gamma * pl * frcp(rl));
+  const float cr = fsqrt(ph.gamma * pr * frcp(rr));
   const float El = pl * ph.inv_gm1 + (0.5f * rl) * vsq<DIM>(wl);
   const float Er = pr * ph.inv_gm1 + (0.5f * rr) * vsq<DIM>(wr);
   const float sp = fmaxf(fmaxf(ul + cl, ur + cr), 0.f);
   const float sm = fminf(fminf(ul - cl, ur - cr), 0.f);
-  const float inv = 1.0f / (sp - sm);
+  const float inv = frcp(sp - sm);
   const float smsp = sm * sp;
   f[0] = ((sp * (ul * rl) - sm * (ur * rr)) + smsp * (rr - rl)) * inv;
 #pragma unroll
@@ -141,8 +155,9 @@ __device__ __forceinline__ void flux_hllc(const float* wl, const float* wr, floa
   constexpr int P = DIM + 1, NV = DIM + 2;
   const float rl = wl[0], ul = wl[AX], pl = wl[P];
   const float rr = wr[0], ur = wr[AX], pr = wr[P];
-  const float cl = sqrtf(ph.gamma * pl / rl);
-  const float cr = sqrtf(ph.gamma * pr / rr);
+  const float irl = frcp(rl), irr = frcp(rr);
+  const float cl = fsqrt(ph.gamma * pl * irl);
+  const float cr = fsqrt(ph.gamma * pr * irr);
   float FL[NV], FR[NV], UL[NV], UR[NV];
   const float El = pl * ph.inv_gm1 + (0.5f * rl) * vsq<DIM>(wl);
   const float Er = pr * ph.inv_gm1 + (0.5f * rr) * vsq<DIM>(wr);
@@ -161,17 +176,18 @@ __device__ __forceinline__ void flux_hllc(const float* wl, const float* wr, floa
   UR[P] = Er;
   FL[P] = ul * (pl + El);
   FR[P] = ur * (pr + Er);
-  const float sql = sqrtf(rl), sqr = sqrtf(rr);
+  const float sql = fsqrt(rl), sqr = fsqrt(rr);
   const float ssum = sql + sqr;
-  const float u_hat = (sql * ul + sqr * ur) / ssum;
+  const float iss = frcp(ssum);
+  const float u_hat = (sql * ul + sqr * ur) * iss;
   const float du = ur - ul;
-  const float c_hat2 = ((cl * cl) * sql + (cr * cr) * sqr) / ssum + (0.5f * ((sql * sqr) / (ssum * ssum))) * (du * du);
-  const float c_hat = sqrtf(c_hat2);
+  const float c_hat2 = ((cl * cl) * sql + (cr * cr) * sqr) * iss + (0.5f * ((sql * sqr) * (iss * iss))) * (du * du);
+  const float c_hat = fsqrt(c_hat2);
   const float SL = fminf(ul - cl, u_hat - c_hat);
   const float SR = fmaxf(ur + cr, u_hat + c_hat);
-  const float s_star = (((pr - pl) + (rl * ul) * (SL - ul)) - (rr * ur) * (SR - ur)) / (rl * (SL - ul) - rr * (SR - ur));
-  const float fl = (SL - ul) / (SL - s_star);
-  const float fr = (SR - ur) / (SR - s_star);
+  const float s_star = (((pr - pl) + (rl * ul) * (SL - ul)) - (rr * ur) * (SR - ur)) * frcp(rl * (SL - ul) - rr * (SR - ur));
+  const float fl = (SL - ul) * frcp(SL - s_star);
+  const float fr = (SR - ur) * frcp(SR - s_star);
   const float as = fabsf(s_star);
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
@@ -181,8 +197,8 @@ __device__ __forceinline__ void flux_hllc(const float* wl, const float* wr, floa
       usr = rr * s_star;
     }
     if (v == P) {
-      usl = usl + (s_star - ul) * (rl * s_star + pl / (SL - ul));
-      usr = usr + (s_star - ur) * (rr * s_star + pr / (SR - ur));
+      usl = usl + (s_star - ul) * (rl * s_star + pl * frcp(SL - ul));
+      usr = usr + (s_star - ur) * (rr * s_star + pr * frcp(SR - ur));
     }
     usl *= fl;
     usr *= fr;
@@ -202,32 +218,58 @@ __device__ __forceinline__ void riemann(const float* wl, const float* wr, float*
 
 enum { SRC_PLAIN = 0, SRC_MAGFIX = 1 };
 
-// Primitive gas state at one cell. SRC_PLAIN reads the NV gas rows. SRC_MAGFIX
-// reads (rho, v0, p) of the gas state entering the magnetic sub-step plus the
-// converged velocity vn (slots 5..7) and applies the sub-step's write-back:
-// v <- vn and the energy fix with gamma = 5/3 (_magnetic_field_update.py:208-240).
+// Primitive gas state at one cell, in two halves so that loads can be issued
+// a plane ahead of their use: load_raw fetches the fields, derive_w forms W.
+// SRC_PLAIN reads the NV gas rows. SRC_MAGFIX reads (rho, v0, p) of the gas
+// state entering the magnetic sub-step plus the converged velocity vn (slots
+// 5..7) and applies the sub-step's write-back: v <- vn and the energy fix with
+// gamma = 5/3 (_magnetic_field_update.py:208-240).
 template <int DIM, int MODE>
-__device__ __forceinline__ void load_w(const float* const* s, long c, float* w) {
+struct Raw {
+  static constexpr int N = MODE == SRC_PLAIN ? DIM + 2 : 8;
+  float f[N];
+};
+
+template <int DIM, int MODE>
+__device__ __forceinline__ void load_raw(const float* const* s, long c, Raw<DIM, MODE>& r) {
+  if (MODE == SRC_PLAIN) {
+#pragma unroll
+    for (int v = 0; v < DIM + 2; ++v) r.f[v] = s[v][c];
+  } else {
+#pragma unroll
+    for (int v = 0; v < 8; ++v) r.f[v] = (DIM == 2 && v == 4) ? 0.f : s[v][c];
+  }
+}
+
+template <int DIM, int MODE>
+__device__ __forceinline__ void derive_w(const Raw<DIM, MODE>& r, float* w) {
   constexpr int NV = DIM + 2, P = DIM + 1;
   if (MODE == SRC_PLAIN) {
 #pragma unroll
-    for (int v = 0; v < NV; ++v) w[v] = s[v][c];
+    for (int v = 0; v < NV; ++v) w[v] = r.f[v];
   } else {
-    const float rho = s[0][c], p = s[P][c];
-    const float v0x = s[1][c], v0y = s[2][c], v0z = DIM == 3 ? s[3][c] : 0.f;
-    const float vnx = s[5][c], vny = s[6][c], vnz = s[7][c];
-    const float n0 = sqrtf((v0x * v0x + v0y * v0y) + v0z * v0z);
-    const float nn = sqrtf((vnx * vnx + vny * vny) + vnz * vnz);
+    const float rho = r.f[0], p = r.f[P];
+    const float v0x = r.f[1], v0y = r.f[2], v0z = DIM == 3 ? r.f[3] : 0.f;
+    const float vnx = r.f[5], vny = r.f[6], vnz = r.f[7];
+    const float n0 = fsqrt((v0x * v0x + v0y * v0y) + v0z * v0z);
+    const float nn = fsqrt((vnx * vnx + vny * vny) + vnz * vnz);
     const float hr = 0.5f * rho;
-    const float E = p / G53M1 + hr * (n0 * n0);
+    const float E = p * (1.0f / G53M1) + hr * (n0 * n0);
     const float E2 = (E - hr * (n0 * n0)) + hr * (nn * nn);
-    const float e = E2 / rho - 0.5f * (nn * nn);
+    const float e = E2 * frcp(rho) - 0.5f * (nn * nn);
     w[0] = rho;
     w[1] = vnx;
     w[2] = vny;
     if (DIM == 3) w[3] = vnz;
     w[P] = (G53M1 * rho) * e;
   }
+}
+
+template <int DIM, int MODE>
+__device__ __forceinline__ void load_w(const float* const* s, long c, float* w) {
+  Raw<DIM, MODE> r;
+  load_raw<DIM, MODE>(s, c, r);
+  derive_w<DIM, MODE>(r, w);
 }
 
 struct StageArgs {
@@ -263,21 +305,51 @@ __device__ __forceinline__ unsigned wave_max_u(unsigned v) {
   return v;
 }
 
+// Max over the block of one value per thread; the result is valid in thread 0.
+// Every thread of the block must call it (it contains a barrier).
+template <int NW>
+__device__ __forceinline__ unsigned block_max_u(unsigned v, unsigned* sred) {
+  v = wave_max_u(v);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) sred[w] = v;
+  __syncthreads();
+  unsigned m = 0;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < NW; ++i) m = sred[i] > m ? sred[i] : m;
+  }
+  return m;
+}
+
+__device__ __forceinline__ unsigned slots_max(const unsigned* p) {
+  unsigned m = 0;
+  for (int i = 0; i < NSLOT; ++i) m = p[i] > m ? p[i] : m;
+  return m;
+}
+
 // One unsplit forward-Euler stage (evolve_state.py:338-414), optionally with
 // the RK2 Heun average (evolve_state.py:456-469).
+//
+// Block = (TA + 1) wavefronts of 64 lanes. Waves 0..TA-1 own the TA x 64 tile
+// cells of the current plane (lane = contiguous axis B, wave = row along A);
+// the extra wave only loads halo and computes interface fluxes, so that the
+// (TA+1)*64 A-interfaces and TA*65 B-interfaces of a plane take exactly two
+// rounds of the block. In 3D the block marches along x over lx planes with the
+// x-stencil of its own column in registers.
 template <int DIM, int TA, int SRC, bool HEUN, int W0M, int RS>
-__global__ __launch_bounds__(TA * TB) void k_stage(const StageArgs a) {
-  constexpr int NV = DIM + 2, P = DIM + 1, NT = TA * TB, WW = TB + 4;
+__global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(AMX_STAGE_WAVES))) void k_stage(const StageArgs a) {
+  constexpr int NV = DIM + 2, P = DIM + 1, NT = (TA + 1) * TB, WW = TB + 4;
   constexpr int AXA = DIM == 3 ? 2 : 1;  // velocity row of the plane axis A
   constexpr int AXB = DIM == 3 ? 3 : 2;  // velocity row of the plane axis B (contiguous)
+  constexpr int NH = 4 * TB + 4 * TA;    // halo cells of a plane (cross shape)
+  constexpr int NFA = (TA + 1) * TB, NF = NFA + TA * (TB + 1);
   __shared__ float sW[NV][(TA + 4) * WW];
-  __shared__ float sDA[NV][(TA + 2) * TB];
-  __shared__ float sDB[NV][TA * (TB + 2)];
   __shared__ float sFA[NV][(TA + 1) * TB];
   __shared__ float sFB[NV][TA * (TB + 1)];
 
   const Phys ph = a.ph;
   const int tid = threadIdx.x, ta = tid / TB, tb = tid % TB;
+  const bool owner = ta < TA;
   const int bid = swizzle_block(blockIdx.x, gridDim.x);
   const int itb = bid % a.ntb;
   const int rest = bid / a.ntb;
@@ -286,280 +358,184 @@ __global__ __launch_bounds__(TA * TB) void k_stage(const StageArgs a) {
   const int a0 = ita * TA, b0 = itb * TB;
   const long sa = a.nb, sx = (long)a.na * a.nb;
   const int ga = wrapi(a0 + ta, a.na), gb = wrapi(b0 + tb, a.nb);
-  const bool active = (a0 + ta < a.na) && (b0 + tb < a.nb);
-  const float dt = a.dt_scale * (*a.dtp);
-  const float cdt = ph.inv_dx;
+  const long col = (long)ga * sa + gb;
+  const bool active = owner && (a0 + ta < a.na) && (b0 + tb < a.nb);
+  const float coef = ph.inv_dx * (a.dt_scale * (*a.dtp));
+
+  // this thread's halo cell (threads NT-NH .. NT-1), fixed for all planes
+  const int h = tid - (NT - NH);
+  int hr = 0, hc = 0;
+  long hoff = 0;
+  if (h >= 0) {
+    if (h < 4 * TB) {
+      const int q = h / TB;
+      hc = h % TB;
+      hr = q < 2 ? q - 2 : TA + q - 2;
+    } else {
+      const int j = h - 4 * TB;
+      hr = j >> 2;
+      const int q = j & 3;
+      hc = q < 2 ? q - 2 : TB + q - 2;
+    }
+    hoff = (long)wrapi(a0 + hr, a.na) * sa + wrapi(b0 + hc, a.nb);
+  }
+
   int x0 = 0, x1 = 1;
   if (DIM == 3) {
     x0 = icx * a.lx;
     x1 = min(x0 + a.lx, a.nx);
   }
-
   float q0[NV], q1[NV], d0[NV], Fp[NV];
-  if (DIM == 3) {
-    float qm2[NV], qm1[NV], dm1[NV], wl[NV], wr[NV];
-    const long col = (long)ga * sa + gb;
-    load_w<DIM, SRC>(a.src, (long)wrapi(x0 - 2, a.nx) * sx + col, qm2);
-    load_w<DIM, SRC>(a.src, (long)wrapi(x0 - 1, a.nx) * sx + col, qm1);
-    load_w<DIM, SRC>(a.src, (long)x0 * sx + col, q0);
-    load_w<DIM, SRC>(a.src, (long)wrapi(x0 + 1, a.nx) * sx + col, q1);
+  if (owner) {
+    if (DIM == 3) {
+      float qm2[NV], qm1[NV], wl[NV], wr[NV];
+      load_w<DIM, SRC>(a.src, (long)wrapi(x0 - 2, a.nx) * sx + col, qm2);
+      load_w<DIM, SRC>(a.src, (long)wrapi(x0 - 1, a.nx) * sx + col, qm1);
+      load_w<DIM, SRC>(a.src, (long)x0 * sx + col, q0);
+      load_w<DIM, SRC>(a.src, (long)wrapi(x0 + 1, a.nx) * sx + col, q1);
 #pragma unroll
-    for (int v = 0; v < NV; ++v) {
-      dm1[v] = half_slope(qm2[v], qm1[v], q0[v]);
-      d0[v] = half_slope(qm1[v], q0[v], q1[v]);
-      wl[v] = qm1[v] + dm1[v];
-      wr[v] = q0[v] - d0[v];
+      for (int v = 0; v < NV; ++v) {
+        d0[v] = half_slope(qm1[v], q0[v], q1[v]);
+        wl[v] = qm1[v] + half_slope(qm2[v], qm1[v], q0[v]);
+        wr[v] = q0[v] - d0[v];
+      }
+      riemann<DIM, 1, RS>(wl, wr, Fp, ph);
+    } else {
+      load_w<DIM, SRC>(a.src, col, q0);
     }
-    riemann<DIM, 1, RS>(wl, wr, Fp, ph);
-  } else {
-    load_w<DIM, SRC>(a.src, (long)ga * sa + gb, q0);
   }
+
+  // loads for plane x are issued one plane ahead of their use (raw fields in
+  // registers across the previous plane's flux phase)
+  Raw<DIM, SRC> rq, rh;  // own column at x+2, halo cell at x
+  Raw<DIM, W0M> r0;      // Heun centre at x
+  if (owner && DIM == 3) load_raw<DIM, SRC>(a.src, (long)wrapi(x0 + 2, a.nx) * sx + col, rq);
+  if (h >= 0) load_raw<DIM, SRC>(a.src, (DIM == 3 ? (long)x0 * sx : 0) + hoff, rh);
+  // the 8-field energy-fix centre is not prefetched (register pressure)
+  constexpr bool PRE0 = HEUN && W0M == SRC_PLAIN;
+  if (PRE0 && active) load_raw<DIM, W0M>(a.w0, (DIM == 3 ? (long)x0 * sx : 0) + col, r0);
 
   float mx[3] = {0.f, 0.f, 0.f};
   for (int x = x0; x < x1; ++x) {
     const long plane = DIM == 3 ? (long)x * sx : 0;
-    float Fn[NV];
-    if (DIM == 3) {
-      float q2[NV], d1[NV], wl[NV], wr[NV];
-      load_w<DIM, SRC>(a.src, (long)wrapi(x + 2, a.nx) * sx + (long)ga * sa + gb, q2);
+    const long nplane = DIM == 3 ? (long)(x + 1) * sx : 0;
+    const bool more = x + 1 < x1;
+    float Fn[NV], q2[NV], hw[NV];
+    Raw<DIM, W0M> r0c = r0;
+    if (owner && DIM == 3) derive_w<DIM, SRC>(rq, q2);
+    if (h >= 0) derive_w<DIM, SRC>(rh, hw);
+    if (more) {
+      if (owner && DIM == 3) load_raw<DIM, SRC>(a.src, (long)wrapi(x + 3, a.nx) * sx + col, rq);
+      if (h >= 0) load_raw<DIM, SRC>(a.src, nplane + hoff, rh);
+      if (PRE0 && active) load_raw<DIM, W0M>(a.w0, nplane + col, r0);
+    }
+    if (owner) {
+      if (DIM == 3) {
+        float d1[NV], wl[NV], wr[NV];
 #pragma unroll
-      for (int v = 0; v < NV; ++v) {
-        d1[v] = half_slope(q0[v], q1[v], q2[v]);
-        wl[v] = q0[v] + d0[v];
-        wr[v] = q1[v] - d1[v];
+        for (int v = 0; v < NV; ++v) {
+          d1[v] = half_slope(q0[v], q1[v], q2[v]);
+          wl[v] = q0[v] + d0[v];
+          wr[v] = q1[v] - d1[v];
+          d0[v] = d1[v];
+        }
+        riemann<DIM, 1, RS>(wl, wr, Fn, ph);
       }
-      riemann<DIM, 1, RS>(wl, wr, Fn, ph);
-#pragma unroll
-      for (int v = 0; v < NV; ++v) d0[v] = d1[v];
-      // plane fill (own cell) and halo
 #pragma unroll
       for (int v = 0; v < NV; ++v) sW[v][(ta + 2) * WW + tb + 2] = q0[v];
-      constexpr int NH = 4 * TB + 4 * TA;
-      for (int h = tid; h < NH; h += NT) {
-        int r, c;
-        if (h < 4 * TB) {
-          const int q = h / TB;
-          c = h % TB;
-          r = q < 2 ? q - 2 : TA + q - 2;
+    }
+    if (h >= 0) {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) sW[v][(hr + 2) * WW + hc + 2] = hw[v];
+    }
+    __syncthreads();
+    // interface fluxes of the plane; limited slopes fused in (4-point stencils)
+    for (int i = tid; i < NF; i += NT) {
+      float wl[NV], wr[NV], f[NV];
+      if (i < NFA) {
+        const int r = i / TB, c = i % TB;  // interface between rows r-1 and r
+        const int o = (r + 2) * WW + c + 2;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          const float w0 = sW[v][o - 2 * WW], w1 = sW[v][o - WW], w2 = sW[v][o], w3 = sW[v][o + WW];
+          wl[v] = w1 + half_slope(w0, w1, w2);
+          wr[v] = w2 - half_slope(w1, w2, w3);
+        }
+        riemann<DIM, AXA, RS>(wl, wr, f, ph);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) sFA[v][i] = f[v];
+      } else {
+        const int j = i - NFA;
+        const int r = j / (TB + 1), c = j % (TB + 1);  // interface between columns c-1 and c
+        const int o = (r + 2) * WW + c + 2;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          const float w0 = sW[v][o - 2], w1 = sW[v][o - 1], w2 = sW[v][o], w3 = sW[v][o + 1];
+          wl[v] = w1 + half_slope(w0, w1, w2);
+          wr[v] = w2 - half_slope(w1, w2, w3);
+        }
+        riemann<DIM, AXB, RS>(wl, wr, f, ph);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) sFB[v][j] = f[v];
+      }
+    }
+    __syncthreads();
+    if (active) {
+      const long cell = plane + col;
+      float U[NV], W1[NV];
+      cons<DIM>(q0, U, ph);
+      if (DIM == 3) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) U[v] = U[v] + (Fp[v] - Fn[v]) * coef;
+      }
+#pragma unroll
+      for (int v = 0; v < NV; ++v) U[v] = U[v] + (sFA[v][ta * TB + tb] - sFA[v][(ta + 1) * TB + tb]) * coef;
+#pragma unroll
+      for (int v = 0; v < NV; ++v) U[v] = U[v] + (sFB[v][ta * (TB + 1) + tb] - sFB[v][ta * (TB + 1) + tb + 1]) * coef;
+      prim<DIM>(U, W1, ph);
+      if (HEUN) {
+        float W0[NV], U0[NV], U2[NV];
+        if (PRE0) {
+          derive_w<DIM, W0M>(r0c, W0);
         } else {
-          const int j = h - 4 * TB;
-          r = j >> 2;
-          const int q = j & 3;
-          c = q < 2 ? q - 2 : TB + q - 2;
-        }
-        float w[NV];
-        load_w<DIM, SRC>(a.src, plane + (long)wrapi(a0 + r, a.na) * sa + wrapi(b0 + c, a.nb), w);
-#pragma unroll
-        for (int v = 0; v < NV; ++v) sW[v][(r + 2) * WW + c + 2] = w[v];
-      }
-      __syncthreads();
-      // ---- limited slopes along A and B
-      {
-        constexpr int NDA = (TA + 2) * TB, ND = NDA + TA * (TB + 2);
-        for (int i = tid; i < ND; i += NT) {
-          if (i < NDA) {
-            const int r = i / TB - 1, c = i % TB;
-            const int base = (r + 2) * WW + c + 2;
-#pragma unroll
-            for (int v = 0; v < NV; ++v)
-              sDA[v][(r + 1) * TB + c] = half_slope(sW[v][base - WW], sW[v][base], sW[v][base + WW]);
-          } else {
-            const int j = i - NDA;
-            const int r = j / (TB + 2), c = j % (TB + 2) - 1;
-            const int base = (r + 2) * WW + c + 2;
-#pragma unroll
-            for (int v = 0; v < NV; ++v)
-              sDB[v][r * (TB + 2) + c + 1] = half_slope(sW[v][base - 1], sW[v][base], sW[v][base + 1]);
-          }
-        }
-      }
-      __syncthreads();
-      // ---- interface fluxes along A (r-1/2) and B (c-1/2)
-      {
-        constexpr int NFA = (TA + 1) * TB, NF = NFA + TA * (TB + 1);
-        for (int i = tid; i < NF; i += NT) {
-          float wl2[NV], wr2[NV], f[NV];
-          if (i < NFA) {
-            const int r = i / TB, c = i % TB;
-#pragma unroll
-            for (int v = 0; v < NV; ++v) {
-              wl2[v] = sW[v][(r + 1) * WW + c + 2] + sDA[v][r * TB + c];
-              wr2[v] = sW[v][(r + 2) * WW + c + 2] - sDA[v][(r + 1) * TB + c];
-            }
-            riemann<DIM, AXA, RS>(wl2, wr2, f, ph);
-#pragma unroll
-            for (int v = 0; v < NV; ++v) sFA[v][r * TB + c] = f[v];
-          } else {
-            const int j = i - NFA;
-            const int r = j / (TB + 1), c = j % (TB + 1);
-#pragma unroll
-            for (int v = 0; v < NV; ++v) {
-              wl2[v] = sW[v][(r + 2) * WW + c + 1] + sDB[v][r * (TB + 2) + c];
-              wr2[v] = sW[v][(r + 2) * WW + c + 2] - sDB[v][r * (TB + 2) + c + 1];
-            }
-            riemann<DIM, AXB, RS>(wl2, wr2, f, ph);
-#pragma unroll
-            for (int v = 0; v < NV; ++v) sFB[v][r * (TB + 1) + c] = f[v];
-          }
-        }
-      }
-      __syncthreads();
-      // ---- update of the own cell
-      if (active) {
-        const long cell = plane + (long)ga * sa + gb;
-        float U[NV], W1[NV];
-        cons<DIM>(q0, U, ph);
-#pragma unroll
-        for (int v = 0; v < NV; ++v) U[v] = U[v] + (cdt * (Fp[v] - Fn[v])) * dt;
-#pragma unroll
-        for (int v = 0; v < NV; ++v) U[v] = U[v] + (cdt * (sFA[v][ta * TB + tb] - sFA[v][(ta + 1) * TB + tb])) * dt;
-#pragma unroll
-        for (int v = 0; v < NV; ++v)
-          U[v] = U[v] + (cdt * (sFB[v][ta * (TB + 1) + tb] - sFB[v][ta * (TB + 1) + tb + 1])) * dt;
-        prim<DIM>(U, W1, ph);
-        if (HEUN) {
-          float W0[NV], U0[NV], U2[NV];
           load_w<DIM, W0M>(a.w0, cell, W0);
-          cons<DIM>(W0, U0, ph);
-          cons<DIM>(W1, U2, ph);
-#pragma unroll
-          for (int v = 0; v < NV; ++v) U[v] = 0.5f * (U0[v] + U2[v]);
-          prim<DIM>(U, W1, ph);
         }
+        cons<DIM>(W0, U0, ph);
+        cons<DIM>(W1, U2, ph);
 #pragma unroll
-        for (int v = 0; v < NV; ++v) a.out[v][cell] = W1[v];
-        if (a.cflmax) {
-          const float cs = sqrtf(ph.gamma * W1[P] / W1[0]);
+        for (int v = 0; v < NV; ++v) U[v] = 0.5f * (U0[v] + U2[v]);
+        prim<DIM>(U, W1, ph);
+      }
 #pragma unroll
-          for (int k = 0; k < DIM; ++k) {
-            const float m = fabsf(W1[1 + k]) + cs;
-            mx[k] = (m != m || m > mx[k]) ? m : mx[k];
-          }
-        }
-        if (a.bsrc[0]) {
+      for (int v = 0; v < NV; ++v) a.out[v][cell] = W1[v];
+      if (a.cflmax) {
+        const float cs = fsqrt(ph.gamma * W1[P] * frcp(W1[0]));
 #pragma unroll
-          for (int k = 0; k < 3; ++k) a.bdst[k][cell] = a.bsrc[k][cell];
+        for (int k = 0; k < DIM; ++k) {
+          const float m = fabsf(W1[1 + k]) + cs;
+          mx[k] = (m != m || m > mx[k]) ? m : mx[k];
         }
       }
+      if (a.bsrc[0]) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) a.bdst[k][cell] = a.bsrc[k][cell];
+      }
+    }
+    if (DIM == 3 && owner) {
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
         q0[v] = q1[v];
         q1[v] = q2[v];
         Fp[v] = Fn[v];
       }
-    } else {
-      // ---------------- 2D: a single plane (axes A = x, B = y)
-#pragma unroll
-      for (int v = 0; v < NV; ++v) sW[v][(ta + 2) * WW + tb + 2] = q0[v];
-      constexpr int NH = 4 * TB + 4 * TA;
-      for (int h = tid; h < NH; h += NT) {
-        int r, c;
-        if (h < 4 * TB) {
-          const int q = h / TB;
-          c = h % TB;
-          r = q < 2 ? q - 2 : TA + q - 2;
-        } else {
-          const int j = h - 4 * TB;
-          r = j >> 2;
-          const int q = j & 3;
-          c = q < 2 ? q - 2 : TB + q - 2;
-        }
-        float w[NV];
-        load_w<DIM, SRC>(a.src, (long)wrapi(a0 + r, a.na) * sa + wrapi(b0 + c, a.nb), w);
-#pragma unroll
-        for (int v = 0; v < NV; ++v) sW[v][(r + 2) * WW + c + 2] = w[v];
-      }
-      __syncthreads();
-      {
-        constexpr int NDA = (TA + 2) * TB, ND = NDA + TA * (TB + 2);
-        for (int i = tid; i < ND; i += NT) {
-          if (i < NDA) {
-            const int r = i / TB - 1, c = i % TB;
-            const int base = (r + 2) * WW + c + 2;
-#pragma unroll
-            for (int v = 0; v < NV; ++v)
-              sDA[v][(r + 1) * TB + c] = half_slope(sW[v][base - WW], sW[v][base], sW[v][base + WW]);
-          } else {
-            const int j = i - NDA;
-            const int r = j / (TB + 2), c = j % (TB + 2) - 1;
-            const int base = (r + 2) * WW + c + 2;
-#pragma unroll
-            for (int v = 0; v < NV; ++v)
-              sDB[v][r * (TB + 2) + c + 1] = half_slope(sW[v][base - 1], sW[v][base], sW[v][base + 1]);
-          }
-        }
-      }
-      __syncthreads();
-      {
-        constexpr int NFA = (TA + 1) * TB, NF = NFA + TA * (TB + 1);
-        for (int i = tid; i < NF; i += NT) {
-          float wl2[NV], wr2[NV], f[NV];
-          if (i < NFA) {
-            const int r = i / TB, c = i % TB;
-#pragma unroll
-            for (int v = 0; v < NV; ++v) {
-              wl2[v] = sW[v][(r + 1) * WW + c + 2] + sDA[v][r * TB + c];
-              wr2[v] = sW[v][(r + 2) * WW + c + 2] - sDA[v][(r + 1) * TB + c];
-            }
-            riemann<DIM, AXA, RS>(wl2, wr2, f, ph);
-#pragma unroll
-            for (int v = 0; v < NV; ++v) sFA[v][r * TB + c] = f[v];
-          } else {
-            const int j = i - NFA;
-            const int r = j / (TB + 1), c = j % (TB + 1);
-#pragma unroll
-            for (int v = 0; v < NV; ++v) {
-              wl2[v] = sW[v][(r + 2) * WW + c + 1] + sDB[v][r * (TB + 2) + c];
-              wr2[v] = sW[v][(r + 2) * WW + c + 2] - sDB[v][r * (TB + 2) + c + 1];
-            }
-            riemann<DIM, AXB, RS>(wl2, wr2, f, ph);
-#pragma unroll
-            for (int v = 0; v < NV; ++v) sFB[v][r * (TB + 1) + c] = f[v];
-          }
-        }
-      }
-      __syncthreads();
-      if (active) {
-        const long cell = (long)ga * sa + gb;
-        float U[NV], W1[NV];
-        cons<DIM>(q0, U, ph);
-#pragma unroll
-        for (int v = 0; v < NV; ++v) U[v] = U[v] + (cdt * (sFA[v][ta * TB + tb] - sFA[v][(ta + 1) * TB + tb])) * dt;
-#pragma unroll
-        for (int v = 0; v < NV; ++v)
-          U[v] = U[v] + (cdt * (sFB[v][ta * (TB + 1) + tb] - sFB[v][ta * (TB + 1) + tb + 1])) * dt;
-        prim<DIM>(U, W1, ph);
-        if (HEUN) {
-          float W0[NV], U0[NV], U2[NV];
-          load_w<DIM, W0M>(a.w0, cell, W0);
-          cons<DIM>(W0, U0, ph);
-          cons<DIM>(W1, U2, ph);
-#pragma unroll
-          for (int v = 0; v < NV; ++v) U[v] = 0.5f * (U0[v] + U2[v]);
-          prim<DIM>(U, W1, ph);
-        }
-#pragma unroll
-        for (int v = 0; v < NV; ++v) a.out[v][cell] = W1[v];
-        if (a.cflmax) {
-          const float cs = sqrtf(ph.gamma * W1[P] / W1[0]);
-#pragma unroll
-          for (int k = 0; k < DIM; ++k) {
-            const float m = fabsf(W1[1 + k]) + cs;
-            mx[k] = (m != m || m > mx[k]) ? m : mx[k];
-          }
-        }
-        if (a.bsrc[0]) {
-#pragma unroll
-          for (int k = 0; k < 3; ++k) a.bdst[k][cell] = a.bsrc[k][cell];
-        }
-      }
     }
   }
   if (a.cflmax) {
+    __shared__ unsigned sred[DIM][TA + 1];
 #pragma unroll
     for (int k = 0; k < DIM; ++k) {
-      const unsigned m = wave_max_u(canon_bits(mx[k]));
-      if ((tid & 63) == 0 && m) atomicMax(&a.cflmax[k], m);
+      const unsigned m = block_max_u<TA + 1>(canon_bits(mx[k]), sred[k]);
+      if (tid == 0 && m) atomicMax(&a.cflmax[k * NSLOT + (blockIdx.x & (NSLOT - 1))], m);
     }
   }
 }
@@ -624,20 +600,27 @@ __device__ __forceinline__ float cdiff(float fp, float fm, float inv_dx) {
 
 // One fixed-point iterate of the implicit-midpoint magnetic sub-step:
 // B' = B0 - dt curl(avgB x avgv), v' = v0 - dt (avgB x curl avgB) / rho,
-// with the residual max(|Bk - B'|, |vk - v'|) reduced into ctl->res[k]
+// with the residual max(|Bk - B'|, |vk - v'|) reduced into ctl->resp[k % 3]
 // (_magnetic_field_update.py:112-165).
 template <int DIM, int TA>
 __global__ __launch_bounds__(TA * TB) void k_mag(const MagArgs a) {
   constexpr int NT = TA * TB, WW = TB + 2, PL = (TA + 2) * WW;
+  const int tid = threadIdx.x, ta = tid / TB, tb = tid % TB;
   if (a.k > 0) {
-    const float r = __uint_as_float(a.ctl->res[a.k - 1]);
-    if (!(r > a.tol) || a.k - 1 >= a.max_iter) return;
+    // while condition of _magnetic_field_update.py:127-141 on iterate k-1
+    if (a.ctl->done) return;
+    const unsigned rb = wave_max_u(a.ctl->resp[(a.k - 1) % 3][tid & 63]);
+    const float r = __uint_as_float(rb);
+    if (!(r > a.tol) || a.k - 1 >= a.max_iter) {
+      if (blockIdx.x == 0 && tid == 0) a.ctl->done = 1;
+      return;
+    }
   }
   __shared__ float sM[2][6][PL];
-  const int tid = threadIdx.x, ta = tid / TB, tb = tid % TB;
-  if (blockIdx.x == 0 && tid == 0) {
-    a.ctl->iters = a.k;
-    a.ctl->res[a.k + 1] = 0u;
+  __shared__ unsigned sred[TA];
+  if (blockIdx.x == 0 && tid < NSLOT) {
+    if (tid == 0) a.ctl->iters = a.k;
+    a.ctl->resp[(a.k + 1) % 3][tid] = 0u;
   }
   const int bid = swizzle_block(blockIdx.x, gridDim.x);
   const int itb = bid % a.ntb;
@@ -670,30 +653,37 @@ __global__ __launch_bounds__(TA * TB) void k_mag(const MagArgs a) {
     mag_load(a, col, rc);
     mag_derive(rc, a.euler, dc);
   }
+  // this thread's halo cell of the plane ring (rows -1, TA; columns -1, TB)
+  constexpr int NH = 2 * TB + 2 * TA;
+  int hr = 0, hc = 0;
+  long hoff = 0;
+  if (tid < NH) {
+    if (tid < 2 * TB) {
+      hr = tid < TB ? -1 : TA;
+      hc = tid % TB;
+    } else {
+      const int j = tid - 2 * TB;
+      hr = j >> 1;
+      hc = (j & 1) ? TB : -1;
+    }
+    hoff = (long)wrapi(a0 + hr, a.na) * sa + wrapi(b0 + hc, a.nb);
+  }
   float res = 0.f;
   int buf = 0;
+  // halo loads run one plane ahead of their use
+  MagRaw th;
+  if (tid < NH) mag_load(a, (DIM == 3 ? (long)x0 * sx : 0) + hoff, th);
   for (int x = x0; x < x1; ++x) {
     const long plane = DIM == 3 ? (long)x * sx : 0;
-    // plane fill: own cell and the halo ring (rows -1, TA; columns -1, TB)
+    // plane fill: own cell and the halo ring
 #pragma unroll
     for (int i = 0; i < 6; ++i) sM[buf][i][(ta + 1) * WW + tb + 1] = dc[i];
-    constexpr int NH = 2 * TB + 2 * TA;
-    for (int h = tid; h < NH; h += NT) {
-      int r, c;
-      if (h < 2 * TB) {
-        r = h < TB ? -1 : TA;
-        c = h % TB;
-      } else {
-        const int j = h - 2 * TB;
-        r = j >> 1;
-        c = (j & 1) ? TB : -1;
-      }
-      MagRaw t;
+    if (tid < NH) {
       float d[6];
-      mag_load(a, plane + (long)wrapi(a0 + r, a.na) * sa + wrapi(b0 + c, a.nb), t);
-      mag_derive(t, a.euler, d);
+      mag_derive(th, a.euler, d);
 #pragma unroll
-      for (int i = 0; i < 6; ++i) sM[buf][i][(r + 1) * WW + c + 1] = d[i];
+      for (int i = 0; i < 6; ++i) sM[buf][i][(hr + 1) * WW + hc + 1] = d[i];
+      if (DIM == 3 && x + 1 < x1) mag_load(a, (long)(x + 1) * sx + hoff, th);
     }
     MagRaw rnn;
     float dpp[6];
@@ -723,7 +713,7 @@ __global__ __launch_bounds__(TA * TB) void k_mag(const MagArgs a) {
         cB[1] = -cdiff(S[2][o + WW], S[2][o - WW], idx);
         cB[2] = cdiff(S[1][o + WW], S[1][o - WW], idx) - cdiff(S[0][o + 1], S[0][o - 1], idx);
       }
-      const float ri = 1.0f / a.rho[cell];
+      const float ri = frcp(a.rho[cell]);
       const float p2[3] = {(dc[1] * cB[2] - dc[2] * cB[1]) * ri, (dc[2] * cB[0] - dc[0] * cB[2]) * ri,
                            (dc[0] * cB[1] - dc[1] * cB[0]) * ri};
       float nb = 0.f, nv = 0.f;
@@ -737,7 +727,7 @@ __global__ __launch_bounds__(TA * TB) void k_mag(const MagArgs a) {
         nb += db * db;
         nv += dv * dv;
       }
-      const float r = fmaxf(sqrtf(nb), sqrtf(nv));
+      const float r = fmaxf(fsqrt(nb), fsqrt(nv));
       res = (r != r || r > res) ? r : res;
     }
     if (DIM == 3) {
@@ -753,8 +743,8 @@ __global__ __launch_bounds__(TA * TB) void k_mag(const MagArgs a) {
     }
     buf ^= 1;
   }
-  const unsigned m = wave_max_u(canon_bits(res));
-  if ((tid & 63) == 0 && m) atomicMax(&a.ctl->res[a.k], m);
+  const unsigned m = block_max_u<TA>(canon_bits(res), sred);
+  if (tid == 0 && m) atomicMax(&a.ctl->resp[a.k % 3][blockIdx.x & (NSLOT - 1)], m);
 }
 
 // Unsplit CFL maxima max(|u_a| + c) per axis (_timestep_estimator.py:118-126).
@@ -763,7 +753,7 @@ __global__ void k_cfl(const float* rho, const float* v0, const float* v1, const 
                       long n, float gamma, unsigned* cflmax) {
   float mx[3] = {0.f, 0.f, 0.f};
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-    const float cs = sqrtf(gamma * p[i] / rho[i]);
+    const float cs = fsqrt(gamma * p[i] * frcp(rho[i]));
     const float u[3] = {v0[i], v1[i], DIM == 3 ? v2[i] : 0.f};
 #pragma unroll
     for (int k = 0; k < DIM; ++k) {
@@ -771,10 +761,11 @@ __global__ void k_cfl(const float* rho, const float* v0, const float* v1, const 
       mx[k] = (m != m || m > mx[k]) ? m : mx[k];
     }
   }
+  __shared__ unsigned sred[DIM][4];
 #pragma unroll
   for (int k = 0; k < DIM; ++k) {
-    const unsigned m = wave_max_u(canon_bits(mx[k]));
-    if ((threadIdx.x & 63) == 0 && m) atomicMax(&cflmax[k], m);
+    const unsigned m = block_max_u<4>(canon_bits(mx[k]), sred[k]);
+    if (threadIdx.x == 0 && m) atomicMax(&cflmax[k * NSLOT + (blockIdx.x & (NSLOT - 1))], m);
   }
 }
 
@@ -791,35 +782,50 @@ struct StepParams {
 
 // dt = C / sum_a(max_a / dx) (_timestep_estimator.py:128), or t_end/num_timesteps
 // (time_integration.py:667-668); clamp to t_end - t (time_integration.py:677-678).
+__device__ __forceinline__ void cfl_reduce(Ctl* c, int dim) {
+  for (int k = 0; k < dim; ++k) {
+    c->cfl[k] = slots_max(c->cflp[k]);
+    for (int i = 0; i < NSLOT; ++i) c->cflp[k][i] = 0u;
+  }
+}
+
 __device__ __forceinline__ float cfl_dt_from(const Ctl* c, float C, float dx, int dim) {
   float s = __uint_as_float(c->cfl[0]) / dx + __uint_as_float(c->cfl[1]) / dx;
   if (dim == 3) s = s + __uint_as_float(c->cfl[2]) / dx;
   return (C * 1.0f) / s;
 }
 
+__device__ __forceinline__ void fixed_point_reset(Ctl* c) {
+  for (int i = 0; i < NSLOT; ++i) c->resp[0][i] = 0u;
+  c->iters = 0;
+  c->done = 0;
+}
+
 __global__ void k_step_begin(Ctl* c, StepParams p) {
   float dt;
-  if (p.fixed)
+  if (p.fixed) {
     dt = p.t_end / (float)p.num_timesteps;
-  else if (p.use_cfl)
+  } else if (p.use_cfl) {
+    cfl_reduce(c, p.dim);
     dt = cfl_dt_from(c, p.C, p.dx, p.dim);
-  else
+  } else {
     dt = c->dt;
+  }
   if (p.exact_end) {
     const float rem = p.t_end - c->t;
     dt = rem < dt ? rem : dt;
   }
   c->dt = dt;
-  c->cfl[0] = c->cfl[1] = c->cfl[2] = c->cfl[3] = 0u;
-  c->res[0] = 0u;
-  c->iters = 0;
+  for (int k = 0; k < 3; ++k)
+    for (int i = 0; i < NSLOT; ++i) c->cflp[k][i] = 0u;
+  fixed_point_reset(c);
 }
 
 __global__ void k_set_dt(Ctl* c, const float* dtp) {
   c->dt = *dtp;
-  c->cfl[0] = c->cfl[1] = c->cfl[2] = c->cfl[3] = 0u;
-  c->res[0] = 0u;
-  c->iters = 0;
+  for (int k = 0; k < 3; ++k)
+    for (int i = 0; i < NSLOT; ++i) c->cflp[k][i] = 0u;
+  fixed_point_reset(c);
 }
 
 __global__ void k_step_end(Ctl* c) {
@@ -827,17 +833,36 @@ __global__ void k_step_end(Ctl* c) {
   c->steps += 1;
 }
 
-__global__ void k_cfl_out(const Ctl* c, float C, float dx, int dim, float* out) {
+__global__ void k_cfl_out(Ctl* c, float C, float dx, int dim, float* out) {
+  cfl_reduce(c, dim);
   *out = cfl_dt_from(c, C, dx, dim);
+}
+
+__global__ void k_zero_cfl(Ctl* c) {
+  for (int k = 0; k < 3; ++k)
+    for (int i = 0; i < NSLOT; ++i) c->cflp[k][i] = 0u;
+}
+
+// Streaming copy, 4 floats per lane per iteration.
+__global__ void k_copy(float4* __restrict__ dst, const float4* __restrict__ src, long n4) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (long)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+__global__ void k_copy_tail(float* __restrict__ dst, const float* __restrict__ src, long from, long n) {
+  const long i = from + threadIdx.x;
+  if (i < n) dst[i] = src[i];
 }
 
 __global__ void k_reset(Ctl* c) {
   c->dt = 0.f;
   c->t = 0.f;
-  c->iters = 0;
   c->steps = 0;
   for (int i = 0; i < 4; ++i) c->cfl[i] = 0u;
-  c->res[0] = 0u;
+  for (int k = 0; k < 3; ++k)
+    for (int i = 0; i < NSLOT; ++i) c->cflp[k][i] = c->resp[k][i] = 0u;
+  c->iters = 0;
+  c->done = 0;
 }
 
 }  // namespace amx
@@ -864,6 +889,10 @@ amx_status fail(amx_status s, const char* fmt, const char* what = "") {
   } while (0)
 
 constexpr int TA_DEF = 8;
+#ifndef AMX_MAG_TA
+#define AMX_MAG_TA 8
+#endif
+constexpr int TA_MAG = AMX_MAG_TA;
 
 }  // namespace
 
@@ -874,7 +903,7 @@ struct amx_fv_ctx {
   Phys ph;
   float dx;
   int na, nb, nxm;  // kernel axes: 3D (ny, nz, nx); 2D (nx, ny, 1)
-  int ntb, nta, ncx, lx;
+  int ntb, nta, nta_mag, ncx, lx;
   float* ws;        // G1 (NV fields) | P0 (6 fields) | P1 (6 fields)
   Ctl* ctl;
   Ctl* hctl;        // pinned host mirror
@@ -946,7 +975,7 @@ static amx_status validate(const amx_fv_config* c) {
   if (c->riemann != AMX_HLL && c->riemann != AMX_HLLC) return fail(AMX_ERR_UNSUPPORTED, "riemann solver must be HLL or HLLC%s");
   if (c->magnetic_integrator != AMX_IMPLICIT_MIDPOINT && c->magnetic_integrator != AMX_IMPLICIT_EULER)
     return fail(AMX_ERR_UNSUPPORTED, "unknown magnetic integrator%s");
-  if (c->max_fixed_point_iterations < 0 || c->max_fixed_point_iterations > MAXRES - 4)
+  if (c->max_fixed_point_iterations < 0 || c->max_fixed_point_iterations > 100000)
     return fail(AMX_ERR_INVALID, "max_fixed_point_iterations out of range%s");
   if (!(c->grid_spacing > 0)) return fail(AMX_ERR_INVALID, "grid_spacing must be positive%s");
   return AMX_OK;
@@ -987,6 +1016,7 @@ extern "C" amx_status amx_fv_create(const amx_fv_config* cfg, amx_fv_ctx** out) 
   }
   c->ntb = (c->nb + TB - 1) / TB;
   c->nta = (c->na + TA_DEF - 1) / TA_DEF;
+  c->nta_mag = (c->na + TA_MAG - 1) / TA_MAG;
   if (cfg->dim == 3) {
     int lx = env_int("AMX_LX", 0);
     if (lx <= 0) {
@@ -1062,7 +1092,7 @@ namespace {
 
 template <int DIM, int SRC, bool HEUN, int W0M>
 hipError_t launch_stage_t(const amx_fv_ctx* c, const StageArgs& a, hipStream_t st) {
-  const dim3 grid(c->ntb * c->nta * c->ncx), block(TA_DEF * TB);
+  const dim3 grid(c->ntb * c->nta * c->ncx), block((TA_DEF + 1) * TB);
   if (c->cfg.riemann == AMX_HLLC)
     hipLaunchKernelGGL((k_stage<DIM, TA_DEF, SRC, HEUN, W0M, AMX_HLLC>), grid, block, 0, st, a);
   else
@@ -1104,7 +1134,7 @@ MagArgs mag_base(const amx_fv_ctx* c, const float* dtp) {
   m.na = c->na;
   m.nb = c->nb;
   m.ntb = c->ntb;
-  m.nta = c->nta;
+  m.nta = c->nta_mag;
   m.ncx = c->ncx;
   m.lx = c->lx;
   m.inv_dx = c->ph.inv_dx;
@@ -1112,11 +1142,11 @@ MagArgs mag_base(const amx_fv_ctx* c, const float* dtp) {
 }
 
 hipError_t launch_mag(const amx_fv_ctx* c, const MagArgs& m, hipStream_t st) {
-  const dim3 grid(c->ntb * c->nta * c->ncx), block(TA_DEF * TB);
+  const dim3 grid(c->ntb * c->nta_mag * c->ncx), block(TA_MAG * TB);
   if (c->cfg.dim == 3)
-    hipLaunchKernelGGL((k_mag<3, TA_DEF>), grid, block, 0, st, m);
+    hipLaunchKernelGGL((k_mag<3, TA_MAG>), grid, block, 0, st, m);
   else
-    hipLaunchKernelGGL((k_mag<2, TA_DEF>), grid, block, 0, st, m);
+    hipLaunchKernelGGL((k_mag<2, TA_MAG>), grid, block, 0, st, m);
   return hipGetLastError();
 }
 
@@ -1126,7 +1156,7 @@ hipError_t launch_cfl(const amx_fv_ctx* c, const float* state, hipStream_t st) {
   for (int v = 0; v < c->NV; ++v) f[v] = state + (long)v * n;
   long blocks = (n + 255) / 256;
   if (blocks > 4096) blocks = 4096;
-  unsigned* out = c->ctl->cfl;
+  unsigned* out = &c->ctl->cflp[0][0];
   if (c->cfg.dim == 3)
     hipLaunchKernelGGL(k_cfl<3>, dim3(blocks), dim3(256), 0, st, f[0], f[1], f[2], f[3], f[4], n, c->cfg.gamma, out);
   else
@@ -1159,7 +1189,7 @@ amx_status do_step(amx_fv_ctx* c, float* state, hipStream_t st, int want_cfl, in
       a2.w0[v] = gas[v];
       a2.out[v] = gas[v];
     }
-    a2.cflmax = want_cfl ? c->ctl->cfl : nullptr;
+    a2.cflmax = want_cfl ? &c->ctl->cflp[0][0] : nullptr;
     e = time_begin(c, st);
     HIPCHK((launch_stage<SRC_PLAIN, true, SRC_PLAIN>(c, a2, st)));
     time_end(c, st, e, CLS_STAGE, 0);
@@ -1231,13 +1261,14 @@ amx_status do_step(amx_fv_ctx* c, float* state, hipStream_t st, int want_cfl, in
       HIPCHK(launch_mag(c, mk, st));
       time_end(c, st, ek, CLS_MAG, k);
     }
-    const size_t nb = offsetof(Ctl, res) + sizeof(unsigned) * (size_t)(klast + 2);
-    HIPCHK(hipMemcpyAsync(c->hctl, c->ctl, nb, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(c->hctl, c->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     K = c->hctl->iters;
     time_resolve(c, K);
+    unsigned rb = 0;
+    for (int i = 0; i < NSLOT; ++i) rb = c->hctl->resp[K % 3][i] > rb ? c->hctl->resp[K % 3][i] : rb;
     float r;
-    memcpy(&r, &c->hctl->res[K], sizeof(float));
+    memcpy(&r, &rb, sizeof(float));
     const bool done = !(r > c->cfg.fixed_point_tol) || K >= maxit;
     if (done || klast >= maxit) break;
     if (K < klast) break;  // a kernel exited early: converged (defensive)
@@ -1269,7 +1300,7 @@ amx_status do_step(amx_fv_ctx* c, float* state, hipStream_t st, int want_cfl, in
     b2.bsrc[k] = Pn[k];
     b2.bdst[k] = Bs[k];
   }
-  b2.cflmax = want_cfl ? c->ctl->cfl : nullptr;
+  b2.cflmax = want_cfl ? &c->ctl->cflp[0][0] : nullptr;
   e = time_begin(c, st);
   HIPCHK((launch_stage<SRC_PLAIN, true, SRC_MAGFIX>(c, b2, st)));
   time_end(c, st, e, CLS_STAGE, 0);
@@ -1282,9 +1313,10 @@ amx_status do_step(amx_fv_ctx* c, float* state, hipStream_t st, int want_cfl, in
 extern "C" amx_status amx_fv_cfl_dt(amx_fv_ctx* c, const float* state, float* d_dt, void* stream) {
   if (!c || !state || !d_dt) return fail(AMX_ERR_INVALID, "null argument%s");
   hipStream_t st = (hipStream_t)stream;
-  HIPCHK(hipMemsetAsync(c->ctl->cfl, 0, sizeof(c->ctl->cfl), st));
+  hipLaunchKernelGGL(k_zero_cfl, dim3(1), dim3(1), 0, st, c->ctl);
+  HIPCHK(hipGetLastError());
   HIPCHK(launch_cfl(c, state, st));
-  hipLaunchKernelGGL(k_cfl_out, dim3(1), dim3(1), 0, st, (const Ctl*)c->ctl, c->cfg.C_cfl, c->dx, c->cfg.dim, d_dt);
+  hipLaunchKernelGGL(k_cfl_out, dim3(1), dim3(1), 0, st, c->ctl, c->cfg.C_cfl, c->dx, c->cfg.dim, d_dt);
   HIPCHK(hipGetLastError());
   return AMX_OK;
 }
@@ -1340,7 +1372,7 @@ extern "C" amx_status amx_fv_time_integration(amx_fv_ctx* c, float* state, const
     HIPCHK(hipGetLastError());
     if (!c->cfg.mhd) {
       // hydro steps have no fixed-point readback: read dt here
-      HIPCHK(hipMemcpyAsync(c->hctl, c->ctl, offsetof(Ctl, res), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipMemcpyAsync(c->hctl, c->ctl, offsetof(Ctl, cflp), hipMemcpyDeviceToHost, st));
       HIPCHK(hipStreamSynchronize(st));
       time_resolve(c, 1 << 30);
       t = c->hctl->t;
@@ -1352,7 +1384,7 @@ extern "C" amx_status amx_fv_time_integration(amx_fv_ctx* c, float* state, const
     if (K > kmax) kmax = K;
     ++steps;
   }
-  HIPCHK(hipMemcpyAsync(c->hctl, c->ctl, offsetof(Ctl, res), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(c->hctl, c->ctl, offsetof(Ctl, cflp), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   time_resolve(c, 1 << 30);
   if (stats) {
@@ -1360,6 +1392,24 @@ extern "C" amx_status amx_fv_time_integration(amx_fv_ctx* c, float* state, const
     stats->t = c->hctl->t;
     stats->fixed_point_iterations = iters;
     stats->max_fixed_point_iterations = kmax;
+  }
+  return AMX_OK;
+}
+
+extern "C" amx_status amx_copy_f32(float* dst, const float* src, size_t n, void* stream) {
+  if ((!dst || !src) && n) return fail(AMX_ERR_INVALID, "null argument%s");
+  if (((uintptr_t)dst | (uintptr_t)src) & 15) return fail(AMX_ERR_INVALID, "pointers must be 16-byte aligned%s");
+  hipStream_t st = (hipStream_t)stream;
+  const long n4 = (long)(n / 4);
+  if (n4) {
+    long blocks = (n4 + 255) / 256;
+    if (blocks > 256 * 32) blocks = 256 * 32;
+    hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, st, (float4*)dst, (const float4*)src, n4);
+    HIPCHK(hipGetLastError());
+  }
+  if (n % 4) {
+    hipLaunchKernelGGL(k_copy_tail, dim3(1), dim3(4), 0, st, dst, src, 4 * n4, (long)n);
+    HIPCHK(hipGetLastError());
   }
   return AMX_OK;
 }

@@ -58,6 +58,27 @@ def cpu_baseline(n: int, steps: int):
     }
 
 
+def stream_copy_gbs(gib: float = 1.0, reps: int = 10):
+    """HBM ceiling on this box: the library's streaming copy kernel on a
+    buffer of `gib` GiB (read + write counted), timed with HIP events."""
+    import torch
+    from astronomix_amd import _native
+    n = int(gib * (1 << 30) // 4)
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    _native.copy_f32(dst, src)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        _native.copy_f32(dst, src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del src, dst
+    return {"GBs": round(2 * n * 4 / (ms * 1e-3) / 1e9, 1), "bytes": 2 * n * 4, "avg_launch_ms": round(ms, 4)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -66,6 +87,7 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--copy-gib", type=float, default=1.0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -159,8 +181,11 @@ def main():
             "k_stage_avg_ms": round(stage_avg_ms, 5),
             "k_stage_launches": tm.stage_launches,
         },
+        "stream_copy": None,
         "cpu_baseline": None,
     }
+    if args.copy_gib > 0:
+        out["stream_copy"] = stream_copy_gbs(args.copy_gib)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(n, args.cpu_steps)
     print(json.dumps(out), flush=True)

@@ -121,6 +121,12 @@ typedef struct amx_fv_timing {
 amx_status amx_fv_set_timing(amx_fv_ctx* ctx, int enable);
 amx_status amx_fv_get_timing(amx_fv_ctx* ctx, amx_fv_timing* out);
 
+/* Plain device copy dst[i] = src[i], i < n (16-byte vector loads/stores).
+ * No counterpart in the reference: the HBM ceiling (STREAM copy) the
+ * benchmark reports next to the kernels, and the known-size kernel against
+ * which rocprofv3 FETCH_SIZE / WRITE_SIZE are calibrated. */
+amx_status amx_copy_f32(float* dst, const float* src, size_t n, void* stream);
+
 /* Static description of the last error of this thread. */
 const char* amx_status_string(amx_status s);
 const char* amx_last_error(void);
