@@ -64,6 +64,22 @@ class AmxFvTiming(ctypes.Structure):
     ]
 
 
+class AmxFdConfig(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int),
+        ("ny", ctypes.c_int),
+        ("nz", ctypes.c_int),
+        ("enforce_positivity", ctypes.c_int),
+        ("gamma", ctypes.c_float),
+        ("C_cfl", ctypes.c_float),
+        ("dt_max", ctypes.c_float),
+        ("minimum_density", ctypes.c_float),
+        ("minimum_pressure", ctypes.c_float),
+        ("pad_", ctypes.c_float),
+        ("grid_spacing", ctypes.c_double),
+    ]
+
+
 # Every symbol include/amx.h declares, with its ctypes signature.
 _P = ctypes.c_void_p
 SIGNATURES = {
@@ -75,6 +91,15 @@ SIGNATURES = {
     "amx_fv_time_integration": (ctypes.c_int, [_P, _P, ctypes.POINTER(AmxFvRun), ctypes.POINTER(AmxFvStats), _P]),
     "amx_fv_set_timing": (ctypes.c_int, [_P, ctypes.c_int]),
     "amx_fv_get_timing": (ctypes.c_int, [_P, ctypes.POINTER(AmxFvTiming)]),
+    "amx_fd_create": (ctypes.c_int, [ctypes.POINTER(AmxFdConfig), ctypes.POINTER(_P)]),
+    "amx_fd_destroy": (ctypes.c_int, [_P]),
+    "amx_fd_workspace_bytes": (ctypes.c_int, [ctypes.POINTER(AmxFdConfig), ctypes.POINTER(ctypes.c_size_t)]),
+    "amx_fd_cfl_dt": (ctypes.c_int, [_P, _P, _P, _P]),
+    "amx_fd_evolve_state": (ctypes.c_int, [_P, _P, _P, _P]),
+    "amx_fd_time_integration": (ctypes.c_int, [_P, _P, ctypes.POINTER(AmxFvRun), ctypes.POINTER(AmxFvStats), _P]),
+    "amx_fd_interface_fields": (ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P]),
+    "amx_fd_set_timing": (ctypes.c_int, [_P, ctypes.c_int]),
+    "amx_fd_get_timing": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_long)]),
     "amx_copy_f32": (ctypes.c_int, [_P, _P, ctypes.c_size_t, _P]),
     "amx_status_string": (ctypes.c_char_p, [ctypes.c_int]),
     "amx_last_error": (ctypes.c_char_p, []),
@@ -173,6 +198,54 @@ class FvContext:
         return st
 
 
+class FdContext:
+    """Owns one amx_fd_ctx (device workspace of the SSPRK(5,4) FD step)."""
+
+    def __init__(self, cfg: AmxFdConfig):
+        require_gpu()
+        self.lib = load_library()
+        self.cfg = cfg
+        h = ctypes.c_void_p()
+        check(self.lib.amx_fd_create(ctypes.byref(cfg), ctypes.byref(h)), "amx_fd_create")
+        self.handle = h
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.amx_fd_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    _stream = staticmethod(FvContext._stream)
+
+    def cfl_dt(self, state, out, stream=None):
+        check(self.lib.amx_fd_cfl_dt(self.handle, ctypes.c_void_p(state.data_ptr()),
+                                     ctypes.c_void_p(out.data_ptr()), self._stream(stream)), "amx_fd_cfl_dt")
+
+    def evolve_state(self, state, dt, stream=None):
+        check(self.lib.amx_fd_evolve_state(self.handle, ctypes.c_void_p(state.data_ptr()),
+                                           ctypes.c_void_p(dt.data_ptr()), self._stream(stream)),
+              "amx_fd_evolve_state")
+
+    def set_timing(self, enable: bool):
+        check(self.lib.amx_fd_set_timing(self.handle, int(bool(enable))), "amx_fd_set_timing")
+
+    def get_timing(self):
+        ms, n = ctypes.c_double(0), ctypes.c_long(0)
+        check(self.lib.amx_fd_get_timing(self.handle, ctypes.byref(ms), ctypes.byref(n)), "amx_fd_get_timing")
+        return float(ms.value), int(n.value)
+
+    def time_integration(self, state, run: AmxFvRun, stream=None) -> AmxFvStats:
+        st = AmxFvStats()
+        check(self.lib.amx_fd_time_integration(self.handle, ctypes.c_void_p(state.data_ptr()), ctypes.byref(run),
+                                               ctypes.byref(st), self._stream(stream)), "amx_fd_time_integration")
+        return st
+
+
 def copy_f32(dst, src, stream=None):
     """dst[:] = src[:] on the device with the library's streaming copy kernel."""
     import torch
@@ -186,12 +259,21 @@ def copy_f32(dst, src, stream=None):
 _ctx_cache = {}
 
 
-def fv_context(cfg: AmxFvConfig) -> FvContext:
-    """Cached context per (device, configuration)."""
+def _cached(kind, cls, cfg):
     import torch
-    key = (torch.cuda.current_device(),) + tuple(getattr(cfg, f) for f, _ in AmxFvConfig._fields_)
+    key = (kind, torch.cuda.current_device()) + tuple(getattr(cfg, f) for f, _ in type(cfg)._fields_)
     ctx = _ctx_cache.get(key)
     if ctx is None:
-        ctx = FvContext(cfg)
+        ctx = cls(cfg)
         _ctx_cache[key] = ctx
     return ctx
+
+
+def fv_context(cfg: AmxFvConfig) -> FvContext:
+    """Cached context per (device, configuration)."""
+    return _cached("fv", FvContext, cfg)
+
+
+def fd_context(cfg: AmxFdConfig) -> FdContext:
+    """Cached FD context per (device, configuration)."""
+    return _cached("fd", FdContext, cfg)

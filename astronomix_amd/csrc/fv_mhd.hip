@@ -32,6 +32,7 @@
 #include <vector>
 
 #include "amx.h"
+#include "common.h"
 
 namespace amx {
 
@@ -39,7 +40,6 @@ constexpr int TB = 64;      // tile width along the contiguous axis (one wavefro
 #ifndef AMX_STAGE_WAVES
 #define AMX_STAGE_WAVES 5   // >= 5 waves/SIMD keeps two 9-wave stage blocks per CU
 #endif
-constexpr int NSLOT = 64;   // partial-max slots (one cache line each is not needed: 64 x 4 B)
 
 // Device-resident step control: step size, time, fixed-point bookkeeping and
 // the CFL maxima. Reductions go through NSLOT partial slots (block b adds into
@@ -68,17 +68,6 @@ struct Phys {
 
 // gamma is hard-coded to 5/3 in the magnetic energy fix (_magnetic_field_update.py:213,221)
 constexpr float G53M1 = 1.6666666f - 1.0f;
-
-__device__ __forceinline__ int wrapi(int i, int n) {
-  while (i < 0) i += n;
-  while (i >= n) i -= n;
-  return i;
-}
-
-// Hardware reciprocal / square root (v_rcp_f32, v_sqrt_f32: <= 1 ulp) for
-// data-dependent divides and roots in the stencil kernels.
-__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
-__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 // limiters.py:13-15: 0.5 (sgn a + sgn b) min(|a|,|b|) is the argument of
 // smaller magnitude when both have the same strict sign, and 0 otherwise --
@@ -285,47 +274,6 @@ struct StageArgs {
   float dt_scale;        // 0.5: MHD half step dt/2; 1: hydro step
   Phys ph;
 };
-
-__device__ __forceinline__ int swizzle_block(int bid, int nblk) {
-  // consecutive logical tiles on the same XCD (8 XCDs, round-robin dispatch)
-  if ((nblk & 7) == 0) return (bid & 7) * (nblk >> 3) + (bid >> 3);
-  return bid;
-}
-
-__device__ __forceinline__ unsigned canon_bits(float m) {
-  return m != m ? 0x7fc00000u : __float_as_uint(m);
-}
-
-__device__ __forceinline__ unsigned wave_max_u(unsigned v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    const unsigned w = (unsigned)__shfl_xor((int)v, o, 64);
-    v = w > v ? w : v;
-  }
-  return v;
-}
-
-// Max over the block of one value per thread; the result is valid in thread 0.
-// Every thread of the block must call it (it contains a barrier).
-template <int NW>
-__device__ __forceinline__ unsigned block_max_u(unsigned v, unsigned* sred) {
-  v = wave_max_u(v);
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  if (lane == 0) sred[w] = v;
-  __syncthreads();
-  unsigned m = 0;
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int i = 0; i < NW; ++i) m = sred[i] > m ? sred[i] : m;
-  }
-  return m;
-}
-
-__device__ __forceinline__ unsigned slots_max(const unsigned* p) {
-  unsigned m = 0;
-  for (int i = 0; i < NSLOT; ++i) m = p[i] > m ? p[i] : m;
-  return m;
-}
 
 // One unsplit forward-Euler stage (evolve_state.py:338-414), optionally with
 // the RK2 Heun average (evolve_state.py:456-469).
@@ -882,11 +830,7 @@ amx_status fail(amx_status s, const char* fmt, const char* what = "") {
   return s;
 }
 
-#define HIPCHK(expr)                                                              \
-  do {                                                                            \
-    hipError_t _e = (expr);                                                       \
-    if (_e != hipSuccess) return fail(AMX_ERR_HIP, "HIP error: %s", hipGetErrorString(_e)); \
-  } while (0)
+#define HIPCHK(expr) AMX_HIPCHK(expr)
 
 constexpr int TA_DEF = 8;
 #ifndef AMX_MAG_TA
@@ -1426,3 +1370,5 @@ extern "C" const char* amx_status_string(amx_status s) {
 }
 
 extern "C" const char* amx_last_error(void) { return g_err; }
+
+amx_status amx_fail(amx_status s, const char* fmt, const char* what) { return fail(s, fmt, what); }

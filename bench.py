@@ -7,6 +7,7 @@ The state stays resident in HBM for the whole timed region. Prints one JSON
 line (rank 0 only; this build runs on one GPU).
 
     python bench.py [--steps 200] [--warmup 10] [--n 256] [--no-cpu-baseline]
+    python bench.py --workload mhd_turbulence_3d_fd [--steps 100]   # FD WENO-5 + CT
 """
 
 from __future__ import annotations
@@ -58,6 +59,125 @@ def cpu_baseline(n: int, steps: int):
     }
 
 
+# FD (WENO-5 + CT + SSPRK(5,4)), per cell of one k_weno launch: reads
+# rho, m (4) + E, B after the centre update (4), writes the 8 interface fluxes
+# and the 2 CT modified fluxes -> 18 fields = 72 B.
+WENO_BYTES_PER_CELL = 72.0
+# Whole FD step, per cell (fields read + written once per pass):
+#   init R11 W22; per stage: prep R12(+2 floors) W4(+2), 3 x weno 18, ct_edge R6 W3,
+#   ct_avg R3 W3, face R9 W6 (final R6 W3), qupd R48 W16 (final R32 W8); finish R11 W11
+FD_STEP_FIELDS = 33 + 4 * (20 + 54 + 9 + 6 + 15 + 64) + (16 + 54 + 9 + 6 + 9 + 40) + 22
+
+
+def cpu_baseline_fd(n: int, steps: int):
+    """Time the FD CPU restatement (oracle/fd_oracle.cpp, float32, OpenMP)
+    on a bounded sample of the turbulence workload."""
+    from oracle import fd
+    from astronomix_amd import workloads as W
+    fd._load()
+    cores = len(os.sched_getaffinity(0))
+    state = W.turbulence_3d(n, seed=0)
+    oc = fd.make_config(state.shape, C_cfl=0.4, t_end=1e9, dt_max=0.001)
+    t0 = time.perf_counter()
+    _, done, _ = fd.time_integration(state, oc, max_steps=steps)
+    el = time.perf_counter() - t0
+    return {
+        "value": round(done * n ** 3 / el / 1e6, 4),
+        "unit": "Mzone-updates/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"mhd_turbulence_3d_fd {n}^3, {done} steps from the initial state, float32 C++/OpenMP "
+                  f"restatement (repo CPU restatement, not reference JAX); cpu={platform.processor() or platform.machine()}",
+        "seconds": round(el, 3),
+    }
+
+
+def bench_fd(args):
+    """FD WENO-5 + CT + SSPRK(5,4) on the 3D turbulence start (rho=1, p=1,
+    Bx=0.1, solenoidal Mach-1 velocity, seed 0), CFL 0.4, dt_max 0.001."""
+    import torch
+    from astronomix_amd import _native
+    from astronomix_amd import workloads as W
+    from astronomix_amd._finite_difference import native_config
+
+    n = args.n
+    state_np, cfg, params = W.workload("mhd_turbulence_3d_fd", n)
+    state = torch.from_numpy(state_np).cuda()
+    ctx = _native.fd_context(native_config(state.shape, cfg, params))
+    if args.warmup > 0:
+        ctx.time_integration(state, _native.AmxFvRun(t_end=float(params.t_end), fixed_timestep=0, num_timesteps=0,
+                                                     exact_end_time=1, max_steps=args.warmup))
+    torch.cuda.synchronize()
+    ctx.set_timing(True)
+    run = _native.AmxFvRun(t_end=float(params.t_end), fixed_timestep=0, num_timesteps=0, exact_end_time=1,
+                           max_steps=args.steps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = ctx.time_integration(state, run)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    weno_ms, weno_n = ctx.get_timing()
+    ctx.set_timing(False)
+    if not bool(torch.isfinite(state).all()):
+        print("non-finite state after the timed steps", file=sys.stderr)
+        sys.exit(1)
+    cells = n ** 3
+    steps = st.steps
+    avg_ms = weno_ms / max(weno_n, 1)
+    achieved = WENO_BYTES_PER_CELL * cells / (avg_ms * 1e-3) / 1e9
+    step_bytes = FD_STEP_FIELDS * 4.0
+    traffic = None
+    tpath = os.path.join(HERE, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        try:
+            traffic = json.load(open(tpath)).get(f"k_weno_{n}")
+        except Exception:
+            traffic = None
+    out = {
+        "metric": "Mzone-updates/s, 3D FD MHD (WENO-5 + CT) 256^3 fp32",
+        "value": round(steps * cells / el / 1e6, 3),
+        "unit": "Mzone-updates/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(el / max(steps, 1) * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"mhd_turbulence_3d_fd_{n}: FD WENO-5 characteristic fluxes + HOW-MHD constrained transport, "
+                        f"SSPRK(5,4), CFL 0.4, dt_max 0.001, periodic, rho=1, p=1, Bx=0.1, solenoidal Mach-1 "
+                        f"velocity in shells |k| in [1,3], seed 0",
+            "grid": [n, n, n],
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_weno",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "bytes_per_cell_per_launch": WENO_BYTES_PER_CELL,
+            "avg_launch_ms": round(avg_ms, 5),
+        },
+        "step_model": {
+            "bytes_per_zone_update": step_bytes,
+            "effective_GBs": round(step_bytes * cells * steps / el / 1e9, 1),
+            "k_weno_avg_ms": round(avg_ms, 5),
+            "k_weno_launches": weno_n,
+            "k_weno_share_of_step": round(weno_ms / (el * 1e3), 4),
+        },
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline_fd(min(n, 128), args.cpu_steps)
+    print(json.dumps(out), flush=True)
+
+
 def stream_copy_gbs(gib: float = 1.0, reps: int = 10):
     """HBM ceiling on this box: the library's streaming copy kernel on a
     buffer of `gib` GiB (read + write counted), timed with HIP events."""
@@ -88,11 +208,15 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--copy-gib", type=float, default=1.0)
+    ap.add_argument("--workload", default="mhd_blast_3d", choices=["mhd_blast_3d", "mhd_turbulence_3d_fd"])
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "mhd_turbulence_3d_fd":
+        return bench_fd(args)
 
     import torch
     from astronomix_amd import _native

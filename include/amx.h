@@ -121,6 +121,61 @@ typedef struct amx_fv_timing {
 amx_status amx_fv_set_timing(amx_fv_ctx* ctx, int enable);
 amx_status amx_fv_get_timing(amx_fv_ctx* ctx, amx_fv_timing* out);
 
+/* ---------------------------------------------------------------------------
+ * Finite-difference MHD (config.solver_mode == FINITE_DIFFERENCE): 3D only,
+ * state (11, nx, ny, nz) = [rho, vx, vy, vz, p, Bx, By, Bz, bx, by, bz] where
+ * rows 5-7 are cell-centred B and rows 8-10 the face fields at i+1/2 of x, y, z
+ * (registered_variables.py:188-198). Characteristic WENO-5 fluxes, HOW-MHD
+ * constrained transport, SSPRK(5,4). The run/stats structs are the FV ones.
+ * ------------------------------------------------------------------------- */
+
+/* SimulationConfig / SimulationParams fields the FD step reads
+ * (simulation_config.py:216, simulation_params.py:29-41). */
+typedef struct amx_fd_config {
+  int nx, ny, nz;                 /* cells per axis, each >= 6 */
+  int enforce_positivity;         /* config.enforce_positivity */
+  float gamma;                    /* params.gamma */
+  float C_cfl;                    /* params.C_cfl */
+  float dt_max;                   /* params.dt_max (0.001 by default) */
+  float minimum_density;          /* params.minimum_density */
+  float minimum_pressure;         /* params.minimum_pressure */
+  float pad_;
+  double grid_spacing;            /* config.grid_spacing */
+} amx_fd_config;
+
+typedef struct amx_fd_ctx amx_fd_ctx;
+
+/* Creates an FD context on the current HIP device and allocates its
+ * workspace (SSPRK registers, interface fluxes, CT edge fields). */
+amx_status amx_fd_create(const amx_fd_config* cfg, amx_fd_ctx** out);
+amx_status amx_fd_destroy(amx_fd_ctx* ctx);
+amx_status amx_fd_workspace_bytes(const amx_fd_config* cfg, size_t* bytes);
+
+/* min(C_cfl * dx / sum_d max|lambda_d|, dt_max) over the 7 MHD wave speeds,
+ * written to d_dt. Replaces _cfl_time_step_fd
+ * (_finite_difference/_timestep_estimation/_timestep_estimator.py:26-91). */
+amx_status amx_fd_cfl_dt(amx_fd_ctx* ctx, const float* state, float* d_dt, void* stream);
+
+/* One SSPRK(5,4) step in place with dt read from the device float d_dt.
+ * Replaces _evolve_state_fd (_finite_difference/_state_evolution/_evolve_state.py:25-72). */
+amx_status amx_fd_evolve_state(amx_fd_ctx* ctx, float* state, const float* d_dt, void* stream);
+
+/* Device-resident FD step loop, in place (time_integration.py:629-801, FD
+ * branch, snapshots off). stats->fixed_point_iterations is always 0. */
+amx_status amx_fd_time_integration(amx_fd_ctx* ctx, float* state, const amx_fv_run* run,
+                                   amx_fv_stats* stats, void* stream);
+
+/* Face fields from cell-centred B: b_d = (-B[i-1] + 9 B[i] + 9 B[i+1] - B[i+2]) / 16
+ * along d, periodic; B and b are (3, nx, ny, nz) and must not alias.
+ * Replaces initialize_interface_fields
+ * (_finite_difference/_magnetic_update/_constrained_transport.py:201-213). */
+amx_status amx_fd_interface_fields(const float* B, float* b, int nx, int ny, int nz, void* stream);
+
+/* Event timing of the WENO flux launches (the dominant kernel class).
+ * No counterpart in the reference; used by the benchmark harness. */
+amx_status amx_fd_set_timing(amx_fd_ctx* ctx, int enable);
+amx_status amx_fd_get_timing(amx_fd_ctx* ctx, double* weno_ms, long* weno_launches);
+
 /* Plain device copy dst[i] = src[i], i < n (16-byte vector loads/stores).
  * No counterpart in the reference: the HBM ceiling (STREAM copy) the
  * benchmark reports next to the kernels, and the known-size kernel against
