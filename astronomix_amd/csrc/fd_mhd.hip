@@ -1101,7 +1101,8 @@ extern "C" amx_status amx_fd_destroy(amx_fd_ctx* c) {
   return AMX_OK;
 }
 
-namespace {
+namespace amx {
+namespace fd {
 
 struct CflArgs {
   const float* w[8];
@@ -1112,7 +1113,7 @@ struct CflArgs {
 
 // _cfl_time_step_fd maxima: per direction, max over cells and the 7 modes of
 // |lambda| of the conserved state of w (_timestep_estimator.py(FD):25-87)
-__global__ void k_cfl_args(const CflArgs a) {
+__global__ void k_cfl(const CflArgs a) {
   __shared__ unsigned sred[3][4];
   const Grid g = a.g;
   const Phys ph = a.ph;
@@ -1145,13 +1146,18 @@ __global__ void k_cfl_args(const CflArgs a) {
   }
 }
 
+}  // namespace fd
+}  // namespace amx
+
+namespace {
+
 hipError_t fd_cfl(amx_fd_ctx* c, const float* state, hipStream_t st) {
   CflArgs a;
   for (int v = 0; v < 8; ++v) a.w[v] = state + (long)v * c->g.nc;
   a.g = c->g;
   a.ph = c->ph;
   a.ctl = c->ctl;
-  hipLaunchKernelGGL(k_cfl_args, fd_grid(c), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(k_cfl, fd_grid(c), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 
