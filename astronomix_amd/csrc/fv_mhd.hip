@@ -122,9 +122,11 @@ __device__ __forceinline__ void flux_hll(const float* wl, const float* wr, float
   const float Er = pr * ph.inv_gm1 + (0.5f * rr) * vsq<DIM>(wr);
   const float sp = fmaxf(fmaxf(ul + cl, ur + cr), 0.f);
   const float sm = fminf(fminf(ul - cl, ur - cr), 0.f);
+  // (S+ F_L - S- F_R + S- S+ (U_R - U_L)) / (S+ - S-) with the three weights
+  // folded once: al F_L + ar F_R + as (U_R - U_L)
   const float inv = frcp(sp - sm);
-  const float smsp = sm * sp;
-  f[0] = ((sp * (ul * rl) - sm * (ur * rr)) + smsp * (rr - rl)) * inv;
+  const float al = sp * inv, ar = -(sm * inv), as = (sm * sp) * inv;
+  f[0] = (al * (ul * rl) + ar * (ur * rr)) + as * (rr - rl);
 #pragma unroll
   for (int i = 1; i <= DIM; ++i) {
     const float ml = rl * wl[i], mr = rr * wr[i];
@@ -133,9 +135,9 @@ __device__ __forceinline__ void flux_hll(const float* wl, const float* wr, float
       fl += pl;
       fr += pr;
     }
-    f[i] = ((sp * fl - sm * fr) + smsp * (mr - ml)) * inv;
+    f[i] = (al * fl + ar * fr) + as * (mr - ml);
   }
-  f[P] = ((sp * (ul * (pl + El)) - sm * (ur * (pr + Er))) + smsp * (Er - El)) * inv;
+  f[P] = (al * (ul * (pl + El)) + ar * (ur * (pr + Er))) + as * (Er - El);
 }
 
 // HLLC interface flux, central form (_finite_volume/_riemann_solver/hll.py:113-261).
@@ -291,6 +293,8 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
   constexpr int AXB = DIM == 3 ? 3 : 2;  // velocity row of the plane axis B (contiguous)
   constexpr int NH = 4 * TB + 4 * TA;    // halo cells of a plane (cross shape)
   constexpr int NFA = (TA + 1) * TB, NF = NFA + TA * (TB + 1);
+  static_assert(NFA == NT && NF - NFA <= NT, "flux rounds assume (TA+1)*TB threads");
+  static_assert(NH <= NT, "every halo cell of a plane needs its own thread (TA >= 4)");
   __shared__ float sW[NV][(TA + 4) * WW];
   __shared__ float sFA[NV][(TA + 1) * TB];
   __shared__ float sFB[NV][TA * (TB + 1)];
@@ -397,35 +401,36 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
       for (int v = 0; v < NV; ++v) sW[v][(hr + 2) * WW + hc + 2] = hw[v];
     }
     __syncthreads();
-    // interface fluxes of the plane; limited slopes fused in (4-point stencils)
-    for (int i = tid; i < NF; i += NT) {
+    // interface fluxes of the plane; limited slopes fused in (4-point
+    // stencils). (TA+1)*64 A-interfaces == one per thread; the TA*65
+    // B-interfaces take a second, partial round.
+    {
       float wl[NV], wr[NV], f[NV];
-      if (i < NFA) {
-        const int r = i / TB, c = i % TB;  // interface between rows r-1 and r
-        const int o = (r + 2) * WW + c + 2;
+      const int r = tid / TB, c = tid % TB;  // interface between rows r-1 and r
+      const int o = (r + 2) * WW + c + 2;
 #pragma unroll
-        for (int v = 0; v < NV; ++v) {
-          const float w0 = sW[v][o - 2 * WW], w1 = sW[v][o - WW], w2 = sW[v][o], w3 = sW[v][o + WW];
-          wl[v] = w1 + half_slope(w0, w1, w2);
-          wr[v] = w2 - half_slope(w1, w2, w3);
-        }
-        riemann<DIM, AXA, RS>(wl, wr, f, ph);
-#pragma unroll
-        for (int v = 0; v < NV; ++v) sFA[v][i] = f[v];
-      } else {
-        const int j = i - NFA;
-        const int r = j / (TB + 1), c = j % (TB + 1);  // interface between columns c-1 and c
-        const int o = (r + 2) * WW + c + 2;
-#pragma unroll
-        for (int v = 0; v < NV; ++v) {
-          const float w0 = sW[v][o - 2], w1 = sW[v][o - 1], w2 = sW[v][o], w3 = sW[v][o + 1];
-          wl[v] = w1 + half_slope(w0, w1, w2);
-          wr[v] = w2 - half_slope(w1, w2, w3);
-        }
-        riemann<DIM, AXB, RS>(wl, wr, f, ph);
-#pragma unroll
-        for (int v = 0; v < NV; ++v) sFB[v][j] = f[v];
+      for (int v = 0; v < NV; ++v) {
+        const float w0 = sW[v][o - 2 * WW], w1 = sW[v][o - WW], w2 = sW[v][o], w3 = sW[v][o + WW];
+        wl[v] = w1 + half_slope(w0, w1, w2);
+        wr[v] = w2 - half_slope(w1, w2, w3);
       }
+      riemann<DIM, AXA, RS>(wl, wr, f, ph);
+#pragma unroll
+      for (int v = 0; v < NV; ++v) sFA[v][tid] = f[v];
+    }
+    if (tid < NF - NFA) {
+      float wl[NV], wr[NV], f[NV];
+      const int r = tid / (TB + 1), c = tid % (TB + 1);  // interface between columns c-1 and c
+      const int o = (r + 2) * WW + c + 2;
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        const float w0 = sW[v][o - 2], w1 = sW[v][o - 1], w2 = sW[v][o], w3 = sW[v][o + 1];
+        wl[v] = w1 + half_slope(w0, w1, w2);
+        wr[v] = w2 - half_slope(w1, w2, w3);
+      }
+      riemann<DIM, AXB, RS>(wl, wr, f, ph);
+#pragma unroll
+      for (int v = 0; v < NV; ++v) sFB[v][tid] = f[v];
     }
     __syncthreads();
     if (active) {
@@ -832,7 +837,14 @@ amx_status fail(amx_status s, const char* fmt, const char* what = "") {
 
 #define HIPCHK(expr) AMX_HIPCHK(expr)
 
-constexpr int TA_DEF = 8;
+// Tile rows of k_stage. The block is TA + 1 wavefronts; TA = 7 gives 8
+// waves, two per SIMD, so the barrier-separated phases of a block are
+// balanced over the CU's 4 SIMDs (7- and 9-wave blocks leave one SIMD with
+// an extra wave that every barrier waits for: 6-40 % slower at 256^3).
+#ifndef AMX_STAGE_TA
+#define AMX_STAGE_TA 7
+#endif
+constexpr int TA_DEF = AMX_STAGE_TA;
 #ifndef AMX_MAG_TA
 #define AMX_MAG_TA 8
 #endif
@@ -847,7 +859,7 @@ struct amx_fv_ctx {
   Phys ph;
   float dx;
   int na, nb, nxm;  // kernel axes: 3D (ny, nz, nx); 2D (nx, ny, 1)
-  int ntb, nta, nta_mag, ncx, lx;
+  int ntb, nta, nta_mag, ncx, lx, ncx_mag, lx_mag;
   float* ws;        // G1 (NV fields) | P0 (6 fields) | P1 (6 fields)
   Ctl* ctl;
   Ctl* hctl;        // pinned host mirror
@@ -962,20 +974,26 @@ extern "C" amx_status amx_fv_create(const amx_fv_config* cfg, amx_fv_ctx** out) 
   c->nta = (c->na + TA_DEF - 1) / TA_DEF;
   c->nta_mag = (c->na + TA_MAG - 1) / TA_MAG;
   if (cfg->dim == 3) {
-    int lx = env_int("AMX_LX", 0);
-    if (lx <= 0) {
-      // aim for >= ~2048 blocks (8 per CU) while keeping chunks >= 8 planes
-      const long tiles = (long)c->ntb * c->nta;
-      lx = (int)((tiles * cfg->nx) / 2048);
-      if (lx < 8) lx = 8;
-      if (lx > cfg->nx) lx = cfg->nx;
-    }
-    c->lx = lx;
-    c->ncx = (cfg->nx + lx - 1) / lx;
+    // planes per block along x: aim for >= ~`blocks` blocks with chunks of
+    // >= 8 planes, rounded down to a power of two (the stencil kernels reload
+    // 4 planes at each chunk start, so longer chunks save traffic; the
+    // magnetic kernel, HBM-bound, prefers fewer and longer chunks)
+    auto pick = [&](long tiles, const char* env, long blocks) {
+      int lx = env_int(env, 0);
+      if (lx <= 0) {
+        const long target = (tiles * cfg->nx) / blocks;
+        lx = 8;
+        while (2 * lx <= target) lx *= 2;
+      }
+      return lx > cfg->nx ? cfg->nx : lx;
+    };
+    c->lx = pick((long)c->ntb * c->nta, "AMX_LX", 2048);
+    c->lx_mag = pick((long)c->ntb * c->nta_mag, "AMX_LX_MAG", 1024);
   } else {
-    c->lx = 1;
-    c->ncx = 1;
+    c->lx = c->lx_mag = 1;
   }
+  c->ncx = (c->nxm + c->lx - 1) / c->lx;
+  c->ncx_mag = (c->nxm + c->lx_mag - 1) / c->lx_mag;
   size_t bytes = sizeof(float) * (size_t)c->ncell * (c->NV + (cfg->mhd ? 12 : 0));
   if (hipMalloc((void**)&c->ws, bytes) != hipSuccess) {
     delete c;
@@ -1079,14 +1097,14 @@ MagArgs mag_base(const amx_fv_ctx* c, const float* dtp) {
   m.nb = c->nb;
   m.ntb = c->ntb;
   m.nta = c->nta_mag;
-  m.ncx = c->ncx;
-  m.lx = c->lx;
+  m.ncx = c->ncx_mag;
+  m.lx = c->lx_mag;
   m.inv_dx = c->ph.inv_dx;
   return m;
 }
 
 hipError_t launch_mag(const amx_fv_ctx* c, const MagArgs& m, hipStream_t st) {
-  const dim3 grid(c->ntb * c->nta_mag * c->ncx), block(TA_MAG * TB);
+  const dim3 grid(c->ntb * c->nta_mag * c->ncx_mag), block(TA_MAG * TB);
   if (c->cfg.dim == 3)
     hipLaunchKernelGGL((k_mag<3, TA_MAG>), grid, block, 0, st, m);
   else

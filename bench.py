@@ -108,7 +108,7 @@ def bench_fd(args):
         ctx.time_integration(state, _native.AmxFvRun(t_end=float(params.t_end), fixed_timestep=0, num_timesteps=0,
                                                      exact_end_time=1, max_steps=args.warmup))
     torch.cuda.synchronize()
-    ctx.set_timing(True)
+    ctx.set_timing(not args.no_kernel_timing)
     run = _native.AmxFvRun(t_end=float(params.t_end), fixed_timestep=0, num_timesteps=0, exact_end_time=1,
                            max_steps=args.steps)
     torch.cuda.synchronize()
@@ -124,7 +124,7 @@ def bench_fd(args):
     cells = n ** 3
     steps = st.steps
     avg_ms = weno_ms / max(weno_n, 1)
-    achieved = WENO_BYTES_PER_CELL * cells / (avg_ms * 1e-3) / 1e9
+    achieved = WENO_BYTES_PER_CELL * cells / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     step_bytes = FD_STEP_FIELDS * 4.0
     traffic = None
     tpath = os.path.join(HERE, "profiles", "traffic.json")
@@ -174,7 +174,7 @@ def bench_fd(args):
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline_fd(min(n, 128), args.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline_fd(min(n, 128), args.cpu_steps or 6)
     print(json.dumps(out), flush=True)
 
 
@@ -206,9 +206,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=None,
+                    help="CPU baseline sample length (default: 8 FV steps at 256^3, 6 FD steps at 128^3)")
     ap.add_argument("--copy-gib", type=float, default=1.0)
     ap.add_argument("--workload", default="mhd_blast_3d", choices=["mhd_blast_3d", "mhd_turbulence_3d_fd"])
+    ap.add_argument("--no-kernel-timing", action="store_true",
+                    help="time the steps without per-launch HIP events (no roofline figures)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -232,7 +235,7 @@ def main():
     if args.warmup > 0:
         ctx.time_integration(state, run_w)
     torch.cuda.synchronize()
-    ctx.set_timing(True)
+    ctx.set_timing(not args.no_kernel_timing)
     run = _native.AmxFvRun(t_end=float(params.t_end), fixed_timestep=0, num_timesteps=0, exact_end_time=1,
                            max_steps=args.steps)
     torch.cuda.synchronize()
@@ -257,7 +260,7 @@ def main():
         dom, avg_ms, per_cell = "k_mag", mag_avg_ms, MAG_BYTES_PER_CELL
     else:
         dom, avg_ms, per_cell = "k_stage", stage_avg_ms, STAGE_BYTES_PER_CELL
-    achieved = per_cell * cells / (avg_ms * 1e-3) / 1e9
+    achieved = per_cell * cells / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     traffic = None
     tpath = os.path.join(HERE, "profiles", "traffic.json")
     if os.path.exists(tpath):
@@ -311,7 +314,7 @@ def main():
     if args.copy_gib > 0:
         out["stream_copy"] = stream_copy_gbs(args.copy_gib)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(n, args.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(n, args.cpu_steps or 8)
     print(json.dumps(out), flush=True)
 
 
