@@ -64,6 +64,17 @@ class AmxFvTiming(ctypes.Structure):
     ]
 
 
+class AmxFvSlotStatus(ctypes.Structure):
+    _fields_ = [
+        ("live", ctypes.c_int),
+        ("finished", ctypes.c_int),
+        ("steps", ctypes.c_long),
+        ("t", ctypes.c_double),
+        ("t_end", ctypes.c_double),
+        ("fixed_point_iterations", ctypes.c_long),
+    ]
+
+
 class AmxFdConfig(ctypes.Structure):
     _fields_ = [
         ("nx", ctypes.c_int),
@@ -91,6 +102,17 @@ SIGNATURES = {
     "amx_fv_time_integration": (ctypes.c_int, [_P, _P, ctypes.POINTER(AmxFvRun), ctypes.POINTER(AmxFvStats), _P]),
     "amx_fv_set_timing": (ctypes.c_int, [_P, ctypes.c_int]),
     "amx_fv_get_timing": (ctypes.c_int, [_P, ctypes.POINTER(AmxFvTiming)]),
+    "amx_fv_set_physics": (ctypes.c_int, [_P, ctypes.c_float, ctypes.c_float]),
+    "amx_fv_pool_slot_bytes": (ctypes.c_int, [ctypes.POINTER(AmxFvConfig), ctypes.POINTER(ctypes.c_size_t)]),
+    "amx_fv_pool_create": (ctypes.c_int, [ctypes.POINTER(AmxFvConfig), ctypes.POINTER(AmxFvRun), ctypes.c_int,
+                                          ctypes.POINTER(_P)]),
+    "amx_fv_pool_destroy": (ctypes.c_int, [_P]),
+    "amx_fv_pool_load": (ctypes.c_int, [_P, ctypes.c_int, _P, ctypes.c_float, ctypes.c_float, _P]),
+    "amx_fv_pool_read": (ctypes.c_int, [_P, ctypes.c_int, _P, _P]),
+    "amx_fv_pool_release": (ctypes.c_int, [_P, ctypes.c_int, _P]),
+    "amx_fv_pool_step_launch": (ctypes.c_int, [_P, _P]),
+    "amx_fv_pool_step_finish": (ctypes.c_int, [_P, _P]),
+    "amx_fv_pool_status": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(AmxFvSlotStatus)]),
     "amx_fd_create": (ctypes.c_int, [ctypes.POINTER(AmxFdConfig), ctypes.POINTER(_P)]),
     "amx_fd_destroy": (ctypes.c_int, [_P]),
     "amx_fd_workspace_bytes": (ctypes.c_int, [ctypes.POINTER(AmxFdConfig), ctypes.POINTER(ctypes.c_size_t)]),
@@ -191,10 +213,87 @@ class FvContext:
         check(self.lib.amx_fv_get_timing(self.handle, ctypes.byref(t)), "amx_fv_get_timing")
         return t
 
+    def set_physics(self, gamma: float, C_cfl: float):
+        check(self.lib.amx_fv_set_physics(self.handle, float(gamma), float(C_cfl)), "amx_fv_set_physics")
+        self.cfg.gamma = float(gamma)
+        self.cfg.C_cfl = float(C_cfl)
+
     def time_integration(self, state, run: AmxFvRun, stream=None) -> AmxFvStats:
         st = AmxFvStats()
         check(self.lib.amx_fv_time_integration(self.handle, ctypes.c_void_p(state.data_ptr()), ctypes.byref(run),
                                                ctypes.byref(st), self._stream(stream)), "amx_fv_time_integration")
+        return st
+
+
+def pool_slot_bytes(cfg: AmxFvConfig) -> int:
+    lib = load_library()
+    n = ctypes.c_size_t(0)
+    check(lib.amx_fv_pool_slot_bytes(ctypes.byref(cfg), ctypes.byref(n)), "amx_fv_pool_slot_bytes")
+    return int(n.value)
+
+
+class FvPool:
+    """Owns one amx_fv_pool: `capacity` slots of independent simulations of
+    one grid and scheme, stepped together (continuous batching). All calls
+    go to the pool's own HIP stream."""
+
+    def __init__(self, cfg: AmxFvConfig, capacity: int, run: AmxFvRun = None):
+        import torch
+        require_gpu()
+        self.lib = load_library()
+        self.cfg = cfg
+        self.capacity = int(capacity)
+        if run is None:
+            run = AmxFvRun(t_end=0.0, fixed_timestep=0, num_timesteps=0, exact_end_time=1, max_steps=-1)
+        self.stream = torch.cuda.Stream()
+        h = ctypes.c_void_p()
+        check(self.lib.amx_fv_pool_create(ctypes.byref(cfg), ctypes.byref(run), self.capacity, ctypes.byref(h)),
+              "amx_fv_pool_create")
+        self.handle = h
+        self._s = ctypes.c_void_p(self.stream.cuda_stream)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.amx_fv_pool_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def load(self, slot: int, state, gamma: float, t_end: float):
+        """state: float32 tensor (GPU or CPU) or numpy array in the reference layout."""
+        import numpy as np
+        import torch
+        if isinstance(state, np.ndarray):
+            state = torch.from_numpy(np.ascontiguousarray(state, dtype=np.float32))
+        state = state.contiguous()
+        if state.dtype != torch.float32:
+            raise TypeError("state must be float32")
+        check(self.lib.amx_fv_pool_load(self.handle, int(slot), ctypes.c_void_p(state.data_ptr()),
+                                        float(gamma), float(t_end), self._s), "amx_fv_pool_load")
+        self.stream.synchronize()  # the source may be a temporary
+
+    def read(self, slot: int, out):
+        check(self.lib.amx_fv_pool_read(self.handle, int(slot), ctypes.c_void_p(out.data_ptr()), self._s),
+              "amx_fv_pool_read")
+        self.stream.synchronize()
+        return out
+
+    def release(self, slot: int):
+        check(self.lib.amx_fv_pool_release(self.handle, int(slot), self._s), "amx_fv_pool_release")
+
+    def step_launch(self):
+        check(self.lib.amx_fv_pool_step_launch(self.handle, self._s), "amx_fv_pool_step_launch")
+
+    def step_finish(self):
+        check(self.lib.amx_fv_pool_step_finish(self.handle, self._s), "amx_fv_pool_step_finish")
+
+    def status(self, slot: int) -> AmxFvSlotStatus:
+        st = AmxFvSlotStatus()
+        check(self.lib.amx_fv_pool_status(self.handle, int(slot), ctypes.byref(st)), "amx_fv_pool_status")
         return st
 
 
@@ -270,8 +369,18 @@ def _cached(kind, cls, cfg):
 
 
 def fv_context(cfg: AmxFvConfig) -> FvContext:
-    """Cached context per (device, configuration)."""
-    return _cached("fv", FvContext, cfg)
+    """Cached context per (device, grid and scheme); gamma and C_cfl are set
+    on the cached context when they differ (they are run-time parameters)."""
+    import torch
+    key = ("fv", torch.cuda.current_device()) + tuple(getattr(cfg, f) for f, _ in AmxFvConfig._fields_
+                                                      if f not in ("gamma", "C_cfl"))
+    ctx = _ctx_cache.get(key)
+    if ctx is None:
+        ctx = FvContext(AmxFvConfig.from_buffer_copy(cfg))
+        _ctx_cache[key] = ctx
+    elif ctx.cfg.gamma != cfg.gamma or ctx.cfg.C_cfl != cfg.C_cfl:
+        ctx.set_physics(cfg.gamma, cfg.C_cfl)
+    return ctx
 
 
 def fd_context(cfg: AmxFdConfig) -> FdContext:

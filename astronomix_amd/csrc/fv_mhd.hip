@@ -53,7 +53,9 @@ struct Ctl {
   int iters;   // fixed-point while-body iterations of the current step
   int steps;
   int done;    // fixed point finished (sticky until the next step)
-  int pad[3];
+  int live;    // the slot holds a simulation
+  int run;     // the slot takes part in the current step
+  float t_end; // end time of the slot's simulation
   unsigned cfl[4];              // reduced CFL maxima per axis
   unsigned cflp[3][NSLOT];      // CFL partial maxima
   unsigned resp[3][NSLOT];      // fixed-point residual partials, rotating by iteration % 3
@@ -263,18 +265,28 @@ __device__ __forceinline__ void load_w(const float* const* s, long c, float* w) 
   derive_w<DIM, MODE>(r, w);
 }
 
+// Buffers are given for slot 0; slot s (blockIdx.y + slot0) adds s * sstride
+// floats to every pointer and uses ctl[s] and phs[s]. A single simulation is
+// slot 0 with sstride 0.
 struct StageArgs {
   const float* src[8];   // stencil input (see load_w)
   const float* w0[8];    // Heun: stage-0 state at the cell centre
   float* out[5];         // output gas rows
-  const float* bsrc[3];  // optional copy bsrc -> bdst (final B of the magnetic sub-step)
-  float* bdst[3];
-  const float* dtp;      // device step size
-  unsigned* cflmax;      // optional CFL maxima of the output
+  float* bdst[3];        // optional copy of the final B of the magnetic sub-step
+  // final fixed-point iterate of the slot, chosen on the device by the parity
+  // of its iteration count: v_K replaces src[5..7] (SRC_MAGFIX) and w0[5..7]
+  // (W0M == SRC_MAGFIX), B_K is the copy source
+  const float* selv[2][3];
+  const float* selb[2][3];
+  Ctl* ctl;
+  const Phys* phs;
+  long sstride;
+  int slot0;
+  int want_cfl;          // reduce the CFL maxima of the output into ctl[s].cflp
+  int copy_b;            // copy B_K (selb) into bdst
   int nx, na, nb;        // 3D: (nx, ny, nz); 2D: (1, nx, ny)
   int ntb, nta, ncx, lx;
   float dt_scale;        // 0.5: MHD half step dt/2; 1: hydro step
-  Phys ph;
 };
 
 // One unsplit forward-Euler stage (evolve_state.py:338-414), optionally with
@@ -299,7 +311,30 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
   __shared__ float sFA[NV][(TA + 1) * TB];
   __shared__ float sFB[NV][TA * (TB + 1)];
 
-  const Phys ph = a.ph;
+  const int slot = a.slot0 + blockIdx.y;
+  Ctl* const C = a.ctl + slot;
+  if (!C->run) return;  // block-uniform: empty or finished slot
+  const long so = slot * a.sstride;
+  const int par = C->iters & 1;
+  const float* src[8];
+  const float* w0p[8];
+#pragma unroll
+  for (int v = 0; v < 8; ++v) {
+    src[v] = a.src[v] ? a.src[v] + so : nullptr;
+    w0p[v] = a.w0[v] ? a.w0[v] + so : nullptr;
+  }
+  if (SRC == SRC_MAGFIX) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) src[5 + k] = a.selv[par][k] + so;
+  }
+  if (W0M == SRC_MAGFIX) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) w0p[5 + k] = a.selv[par][k] + so;
+  }
+  float* outp[5];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) outp[v] = a.out[v] + so;
+  const Phys ph = a.phs[slot];
   const int tid = threadIdx.x, ta = tid / TB, tb = tid % TB;
   const bool owner = ta < TA;
   const int bid = swizzle_block(blockIdx.x, gridDim.x);
@@ -312,7 +347,7 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
   const int ga = wrapi(a0 + ta, a.na), gb = wrapi(b0 + tb, a.nb);
   const long col = (long)ga * sa + gb;
   const bool active = owner && (a0 + ta < a.na) && (b0 + tb < a.nb);
-  const float coef = ph.inv_dx * (a.dt_scale * (*a.dtp));
+  const float coef = ph.inv_dx * (a.dt_scale * C->dt);
 
   // this thread's halo cell (threads NT-NH .. NT-1), fixed for all planes
   const int h = tid - (NT - NH);
@@ -341,10 +376,10 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
   if (owner) {
     if (DIM == 3) {
       float qm2[NV], qm1[NV], wl[NV], wr[NV];
-      load_w<DIM, SRC>(a.src, (long)wrapi(x0 - 2, a.nx) * sx + col, qm2);
-      load_w<DIM, SRC>(a.src, (long)wrapi(x0 - 1, a.nx) * sx + col, qm1);
-      load_w<DIM, SRC>(a.src, (long)x0 * sx + col, q0);
-      load_w<DIM, SRC>(a.src, (long)wrapi(x0 + 1, a.nx) * sx + col, q1);
+      load_w<DIM, SRC>(src, (long)wrapi(x0 - 2, a.nx) * sx + col, qm2);
+      load_w<DIM, SRC>(src, (long)wrapi(x0 - 1, a.nx) * sx + col, qm1);
+      load_w<DIM, SRC>(src, (long)x0 * sx + col, q0);
+      load_w<DIM, SRC>(src, (long)wrapi(x0 + 1, a.nx) * sx + col, q1);
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
         d0[v] = half_slope(qm1[v], q0[v], q1[v]);
@@ -353,7 +388,7 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
       }
       riemann<DIM, 1, RS>(wl, wr, Fp, ph);
     } else {
-      load_w<DIM, SRC>(a.src, col, q0);
+      load_w<DIM, SRC>(src, col, q0);
     }
   }
 
@@ -361,11 +396,11 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
   // registers across the previous plane's flux phase)
   Raw<DIM, SRC> rq, rh;  // own column at x+2, halo cell at x
   Raw<DIM, W0M> r0;      // Heun centre at x
-  if (owner && DIM == 3) load_raw<DIM, SRC>(a.src, (long)wrapi(x0 + 2, a.nx) * sx + col, rq);
-  if (h >= 0) load_raw<DIM, SRC>(a.src, (DIM == 3 ? (long)x0 * sx : 0) + hoff, rh);
+  if (owner && DIM == 3) load_raw<DIM, SRC>(src, (long)wrapi(x0 + 2, a.nx) * sx + col, rq);
+  if (h >= 0) load_raw<DIM, SRC>(src, (DIM == 3 ? (long)x0 * sx : 0) + hoff, rh);
   // the 8-field energy-fix centre is not prefetched (register pressure)
   constexpr bool PRE0 = HEUN && W0M == SRC_PLAIN;
-  if (PRE0 && active) load_raw<DIM, W0M>(a.w0, (DIM == 3 ? (long)x0 * sx : 0) + col, r0);
+  if (PRE0 && active) load_raw<DIM, W0M>(w0p, (DIM == 3 ? (long)x0 * sx : 0) + col, r0);
 
   float mx[3] = {0.f, 0.f, 0.f};
   for (int x = x0; x < x1; ++x) {
@@ -377,9 +412,9 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
     if (owner && DIM == 3) derive_w<DIM, SRC>(rq, q2);
     if (h >= 0) derive_w<DIM, SRC>(rh, hw);
     if (more) {
-      if (owner && DIM == 3) load_raw<DIM, SRC>(a.src, (long)wrapi(x + 3, a.nx) * sx + col, rq);
-      if (h >= 0) load_raw<DIM, SRC>(a.src, nplane + hoff, rh);
-      if (PRE0 && active) load_raw<DIM, W0M>(a.w0, nplane + col, r0);
+      if (owner && DIM == 3) load_raw<DIM, SRC>(src, (long)wrapi(x + 3, a.nx) * sx + col, rq);
+      if (h >= 0) load_raw<DIM, SRC>(src, nplane + hoff, rh);
+      if (PRE0 && active) load_raw<DIM, W0M>(w0p, nplane + col, r0);
     }
     if (owner) {
       if (DIM == 3) {
@@ -451,7 +486,7 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
         if (PRE0) {
           derive_w<DIM, W0M>(r0c, W0);
         } else {
-          load_w<DIM, W0M>(a.w0, cell, W0);
+          load_w<DIM, W0M>(w0p, cell, W0);
         }
         cons<DIM>(W0, U0, ph);
         cons<DIM>(W1, U2, ph);
@@ -460,8 +495,8 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
         prim<DIM>(U, W1, ph);
       }
 #pragma unroll
-      for (int v = 0; v < NV; ++v) a.out[v][cell] = W1[v];
-      if (a.cflmax) {
+      for (int v = 0; v < NV; ++v) outp[v][cell] = W1[v];
+      if (a.want_cfl) {
         const float cs = fsqrt(ph.gamma * W1[P] * frcp(W1[0]));
 #pragma unroll
         for (int k = 0; k < DIM; ++k) {
@@ -469,9 +504,9 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
           mx[k] = (m != m || m > mx[k]) ? m : mx[k];
         }
       }
-      if (a.bsrc[0]) {
+      if (a.copy_b) {
 #pragma unroll
-        for (int k = 0; k < 3; ++k) a.bdst[k][cell] = a.bsrc[k][cell];
+        for (int k = 0; k < 3; ++k) a.bdst[k][so + cell] = a.selb[par][k][so + cell];
       }
     }
     if (DIM == 3 && owner) {
@@ -483,12 +518,12 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
       }
     }
   }
-  if (a.cflmax) {
+  if (a.want_cfl) {
     __shared__ unsigned sred[DIM][TA + 1];
 #pragma unroll
     for (int k = 0; k < DIM; ++k) {
       const unsigned m = block_max_u<TA + 1>(canon_bits(mx[k]), sred[k]);
-      if (tid == 0 && m) atomicMax(&a.cflmax[k * NSLOT + (blockIdx.x & (NSLOT - 1))], m);
+      if (tid == 0 && m) atomicMax(&C->cflp[k][blockIdx.x & (NSLOT - 1)], m);
     }
   }
 }
@@ -501,8 +536,9 @@ struct MagArgs {
   const float* vk[3];  // vk[2] == nullptr: vz = 0
   float* Bo[3];
   float* vo[3];
-  const float* dtp;
-  Ctl* ctl;
+  Ctl* ctl;            // per-slot control, slot = slot0 + blockIdx.y
+  long sstride;        // floats between consecutive slots' buffers
+  int slot0;
   int k;         // 0: initial iterate (Bk = B0, vk = v0); k >= 1: while body k
   int max_iter;
   float tol;
@@ -559,22 +595,37 @@ template <int DIM, int TA>
 __global__ __launch_bounds__(TA * TB) void k_mag(const MagArgs a) {
   constexpr int NT = TA * TB, WW = TB + 2, PL = (TA + 2) * WW;
   const int tid = threadIdx.x, ta = tid / TB, tb = tid % TB;
+  const int slot = a.slot0 + blockIdx.y;
+  Ctl* const C = a.ctl + slot;
+  if (!C->run) return;
   if (a.k > 0) {
     // while condition of _magnetic_field_update.py:127-141 on iterate k-1
-    if (a.ctl->done) return;
-    const unsigned rb = wave_max_u(a.ctl->resp[(a.k - 1) % 3][tid & 63]);
+    if (C->done) return;
+    const unsigned rb = wave_max_u(C->resp[(a.k - 1) % 3][tid & 63]);
     const float r = __uint_as_float(rb);
     if (!(r > a.tol) || a.k - 1 >= a.max_iter) {
-      if (blockIdx.x == 0 && tid == 0) a.ctl->done = 1;
+      if (blockIdx.x == 0 && tid == 0) C->done = 1;
       return;
     }
   }
   __shared__ float sM[2][6][PL];
   __shared__ unsigned sred[TA];
   if (blockIdx.x == 0 && tid < NSLOT) {
-    if (tid == 0) a.ctl->iters = a.k;
-    a.ctl->resp[(a.k + 1) % 3][tid] = 0u;
+    if (tid == 0) C->iters = a.k;
+    C->resp[(a.k + 1) % 3][tid] = 0u;
   }
+  const long so = slot * a.sstride;
+  MagArgs m = a;  // slot-local pointers
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    m.B0[i] = a.B0[i] + so;
+    m.Bk[i] = a.Bk[i] + so;
+    m.Bo[i] = a.Bo[i] + so;
+    m.vo[i] = a.vo[i] + so;
+    m.v0[i] = a.v0[i] ? a.v0[i] + so : nullptr;
+    m.vk[i] = a.vk[i] ? a.vk[i] + so : nullptr;
+  }
+  m.rho = a.rho + so;
   const int bid = swizzle_block(blockIdx.x, gridDim.x);
   const int itb = bid % a.ntb;
   const int rest = bid / a.ntb;
@@ -584,7 +635,7 @@ __global__ __launch_bounds__(TA * TB) void k_mag(const MagArgs a) {
   const long sa = a.nb, sx = (long)a.na * a.nb;
   const int ga = wrapi(a0 + ta, a.na), gb = wrapi(b0 + tb, a.nb);
   const bool active = (a0 + ta < a.na) && (b0 + tb < a.nb);
-  const float dt = *a.dtp;
+  const float dt = C->dt;
   const float idx = a.inv_dx;
   const long col = (long)ga * sa + gb;
   int x0 = 0, x1 = 1;
@@ -596,14 +647,14 @@ __global__ __launch_bounds__(TA * TB) void k_mag(const MagArgs a) {
   float dm[6], dc[6], dp[6];
   if (DIM == 3) {
     MagRaw t;
-    mag_load(a, (long)wrapi(x0 - 1, a.nx) * sx + col, t);
+    mag_load(m, (long)wrapi(x0 - 1, a.nx) * sx + col, t);
     mag_derive(t, a.euler, dm);
-    mag_load(a, (long)x0 * sx + col, rc);
+    mag_load(m, (long)x0 * sx + col, rc);
     mag_derive(rc, a.euler, dc);
-    mag_load(a, (long)wrapi(x0 + 1, a.nx) * sx + col, rn);
+    mag_load(m, (long)wrapi(x0 + 1, a.nx) * sx + col, rn);
     mag_derive(rn, a.euler, dp);
   } else {
-    mag_load(a, col, rc);
+    mag_load(m, col, rc);
     mag_derive(rc, a.euler, dc);
   }
   // this thread's halo cell of the plane ring (rows -1, TA; columns -1, TB)
@@ -625,7 +676,7 @@ __global__ __launch_bounds__(TA * TB) void k_mag(const MagArgs a) {
   int buf = 0;
   // halo loads run one plane ahead of their use
   MagRaw th;
-  if (tid < NH) mag_load(a, (DIM == 3 ? (long)x0 * sx : 0) + hoff, th);
+  if (tid < NH) mag_load(m, (DIM == 3 ? (long)x0 * sx : 0) + hoff, th);
   for (int x = x0; x < x1; ++x) {
     const long plane = DIM == 3 ? (long)x * sx : 0;
     // plane fill: own cell and the halo ring
@@ -636,13 +687,13 @@ __global__ __launch_bounds__(TA * TB) void k_mag(const MagArgs a) {
       mag_derive(th, a.euler, d);
 #pragma unroll
       for (int i = 0; i < 6; ++i) sM[buf][i][(hr + 1) * WW + hc + 1] = d[i];
-      if (DIM == 3 && x + 1 < x1) mag_load(a, (long)(x + 1) * sx + hoff, th);
+      if (DIM == 3 && x + 1 < x1) mag_load(m, (long)(x + 1) * sx + hoff, th);
     }
     MagRaw rnn;
     float dpp[6];
     if (DIM == 3) {
       // prefetch the next column cell
-      mag_load(a, (long)wrapi(x + 2, a.nx) * sx + col, rnn);
+      mag_load(m, (long)wrapi(x + 2, a.nx) * sx + col, rnn);
     }
     __syncthreads();
     if (active) {
@@ -666,7 +717,7 @@ __global__ __launch_bounds__(TA * TB) void k_mag(const MagArgs a) {
         cB[1] = -cdiff(S[2][o + WW], S[2][o - WW], idx);
         cB[2] = cdiff(S[1][o + WW], S[1][o - WW], idx) - cdiff(S[0][o + 1], S[0][o - 1], idx);
       }
-      const float ri = frcp(a.rho[cell]);
+      const float ri = frcp(m.rho[cell]);
       const float p2[3] = {(dc[1] * cB[2] - dc[2] * cB[1]) * ri, (dc[2] * cB[0] - dc[0] * cB[2]) * ri,
                            (dc[0] * cB[1] - dc[1] * cB[0]) * ri};
       float nb = 0.f, nv = 0.f;
@@ -674,8 +725,8 @@ __global__ __launch_bounds__(TA * TB) void k_mag(const MagArgs a) {
       for (int i = 0; i < 3; ++i) {
         const float bn = rc.B0[i] - dt * cE[i];
         const float vn = rc.v0[i] - dt * p2[i];
-        a.Bo[i][cell] = bn;
-        a.vo[i][cell] = vn;
+        m.Bo[i][cell] = bn;
+        m.vo[i][cell] = vn;
         const float db = rc.Bk[i] - bn, dv = rc.vk[i] - vn;
         nb += db * db;
         nv += dv * dv;
@@ -696,18 +747,38 @@ __global__ __launch_bounds__(TA * TB) void k_mag(const MagArgs a) {
     }
     buf ^= 1;
   }
-  const unsigned m = block_max_u<TA>(canon_bits(res), sred);
-  if (tid == 0 && m) atomicMax(&a.ctl->resp[a.k % 3][blockIdx.x & (NSLOT - 1)], m);
+  const unsigned mr = block_max_u<TA>(canon_bits(res), sred);
+  if (tid == 0 && mr) atomicMax(&C->resp[a.k % 3][blockIdx.x & (NSLOT - 1)], mr);
 }
 
-// Unsplit CFL maxima max(|u_a| + c) per axis (_timestep_estimator.py:118-126).
+// Unsplit CFL maxima max(|u_a| + c) per axis (_timestep_estimator.py:118-126)
+// of the gas rows of slot slot0 + blockIdx.y into its ctl[].cflp.
+struct CflArgs {
+  const float* w[4];  // rho, v_x, v_y, (v_z): rows of slot 0
+  const float* p;
+  Ctl* ctl;
+  const Phys* phs;
+  long sstride;
+  long n;
+  int slot0;
+};
+
 template <int DIM>
-__global__ void k_cfl(const float* rho, const float* v0, const float* v1, const float* v2, const float* p,
-                      long n, float gamma, unsigned* cflmax) {
+__global__ void k_cfl(const CflArgs a) {
+  const int slot = a.slot0 + blockIdx.y;
+  Ctl* const C = a.ctl + slot;
+  if (!C->live) return;
+  const long so = slot * a.sstride;
+  const float gamma = a.phs[slot].gamma;
+  const float* rho = a.w[0] + so;
+  const float* p = a.p + so;
+  const float* u0 = a.w[1] + so;
+  const float* u1 = a.w[2] + so;
+  const float* u2 = DIM == 3 ? a.w[3] + so : nullptr;
   float mx[3] = {0.f, 0.f, 0.f};
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += (long)gridDim.x * blockDim.x) {
     const float cs = fsqrt(gamma * p[i] * frcp(rho[i]));
-    const float u[3] = {v0[i], v1[i], DIM == 3 ? v2[i] : 0.f};
+    const float u[3] = {u0[i], u1[i], DIM == 3 ? u2[i] : 0.f};
 #pragma unroll
     for (int k = 0; k < DIM; ++k) {
       const float m = fabsf(u[k]) + cs;
@@ -718,19 +789,17 @@ __global__ void k_cfl(const float* rho, const float* v0, const float* v1, const 
 #pragma unroll
   for (int k = 0; k < DIM; ++k) {
     const unsigned m = block_max_u<4>(canon_bits(mx[k]), sred[k]);
-    if (threadIdx.x == 0 && m) atomicMax(&cflmax[k * NSLOT + (blockIdx.x & (NSLOT - 1))], m);
+    if (threadIdx.x == 0 && m) atomicMax(&C->cflp[k][blockIdx.x & (NSLOT - 1)], m);
   }
 }
 
 struct StepParams {
-  float t_end;
   int fixed;
   int num_timesteps;
   int exact_end;
   float C;
   float dx;
   int dim;
-  int use_cfl;  // 1: dt from the CFL maxima; 0: dt already in ctl->dt (host value)
 };
 
 // dt = C / sum_a(max_a / dx) (_timestep_estimator.py:128), or t_end/num_timesteps
@@ -754,34 +823,48 @@ __device__ __forceinline__ void fixed_point_reset(Ctl* c) {
   c->done = 0;
 }
 
-__global__ void k_step_begin(Ctl* c, StepParams p) {
+__device__ __forceinline__ void zero_cflp(Ctl* c) {
+  for (int k = 0; k < 3; ++k)
+    for (int i = 0; i < NSLOT; ++i) c->cflp[k][i] = 0u;
+}
+
+// One thread per slot (slot = blockIdx.x): the step's dt, and whether the
+// slot runs this step (live and not yet at t_end / num_timesteps).
+__global__ void k_step_begin(Ctl* ctl, StepParams p) {
+  Ctl* const c = ctl + blockIdx.x;
+  if (!c->live) {
+    c->run = 0;
+    return;
+  }
+  c->run = p.fixed ? (c->steps < p.num_timesteps) : (c->t < c->t_end);
+  if (!c->run) return;
   float dt;
   if (p.fixed) {
-    dt = p.t_end / (float)p.num_timesteps;
-  } else if (p.use_cfl) {
+    dt = c->t_end / (float)p.num_timesteps;
+  } else {
     cfl_reduce(c, p.dim);
     dt = cfl_dt_from(c, p.C, p.dx, p.dim);
-  } else {
-    dt = c->dt;
   }
   if (p.exact_end) {
-    const float rem = p.t_end - c->t;
+    const float rem = c->t_end - c->t;
     dt = rem < dt ? rem : dt;
   }
   c->dt = dt;
-  for (int k = 0; k < 3; ++k)
-    for (int i = 0; i < NSLOT; ++i) c->cflp[k][i] = 0u;
+  zero_cflp(c);
   fixed_point_reset(c);
 }
 
+// dt given by the caller (one step of _evolve_state_fv)
 __global__ void k_set_dt(Ctl* c, const float* dtp) {
   c->dt = *dtp;
-  for (int k = 0; k < 3; ++k)
-    for (int i = 0; i < NSLOT; ++i) c->cflp[k][i] = 0u;
+  c->run = 1;
+  zero_cflp(c);
   fixed_point_reset(c);
 }
 
-__global__ void k_step_end(Ctl* c) {
+__global__ void k_step_end(Ctl* ctl) {
+  Ctl* const c = ctl + blockIdx.x;
+  if (!c->run) return;
   c->t = c->t + c->dt;
   c->steps += 1;
 }
@@ -791,9 +874,22 @@ __global__ void k_cfl_out(Ctl* c, float C, float dx, int dim, float* out) {
   *out = cfl_dt_from(c, C, dx, dim);
 }
 
-__global__ void k_zero_cfl(Ctl* c) {
+__global__ void k_zero_cfl(Ctl* c) { zero_cflp(c); }
+
+// (Re)start slot blockIdx.x + slot0 at t = 0 (live = 1), or empty it (live = 0).
+__global__ void k_slot_reset(Ctl* ctl, int slot0, float t_end, int live) {
+  Ctl* const c = ctl + slot0 + blockIdx.x;
+  c->dt = 0.f;
+  c->t = 0.f;
+  c->steps = 0;
+  for (int i = 0; i < 4; ++i) c->cfl[i] = 0u;
   for (int k = 0; k < 3; ++k)
-    for (int i = 0; i < NSLOT; ++i) c->cflp[k][i] = 0u;
+    for (int i = 0; i < NSLOT; ++i) c->cflp[k][i] = c->resp[k][i] = 0u;
+  c->iters = 0;
+  c->done = 0;
+  c->live = live;
+  c->run = live;
+  c->t_end = t_end;
 }
 
 // Streaming copy, 4 floats per lane per iteration.
@@ -805,17 +901,6 @@ __global__ void k_copy(float4* __restrict__ dst, const float4* __restrict__ src,
 __global__ void k_copy_tail(float* __restrict__ dst, const float* __restrict__ src, long from, long n) {
   const long i = from + threadIdx.x;
   if (i < n) dst[i] = src[i];
-}
-
-__global__ void k_reset(Ctl* c) {
-  c->dt = 0.f;
-  c->t = 0.f;
-  c->steps = 0;
-  for (int i = 0; i < 4; ++i) c->cfl[i] = 0u;
-  for (int k = 0; k < 3; ++k)
-    for (int i = 0; i < NSLOT; ++i) c->cflp[k][i] = c->resp[k][i] = 0u;
-  c->iters = 0;
-  c->done = 0;
 }
 
 }  // namespace amx
@@ -852,19 +937,36 @@ constexpr int TA_MAG = AMX_MAG_TA;
 
 }  // namespace
 
+// Slots: a context steps `cap` independent simulations of one grid and
+// scheme with the same kernels (the slot is blockIdx.y). A single simulation
+// (amx_fv_create) is one slot working in place on the caller's state; a pool
+// (amx_fv_pool_create) owns cap slots of state + workspace, sstride floats
+// apart.
 struct amx_fv_ctx {
   amx_fv_config cfg;
-  int NV;
+  int NV, NS;       // gas rows; state rows (NV + 3 with MHD)
   long ncell;
-  Phys ph;
   float dx;
   int na, nb, nxm;  // kernel axes: 3D (ny, nz, nx); 2D (nx, ny, 1)
   int ntb, nta, nta_mag, ncx, lx, ncx_mag, lx_mag;
-  float* ws;        // G1 (NV fields) | P0 (6 fields) | P1 (6 fields)
-  Ctl* ctl;
+  int cap;
+  long sstride;     // floats per slot (0 for a single simulation)
+  float* mem;       // pool: cap x [state NS | G1 NV | P0 6 | P1 6]; single: [G1 | P0 | P1]
+  int own_state;
+  Ctl* ctl;         // cap entries
   Ctl* hctl;        // pinned host mirror
+  Phys* dph;        // cap entries
+  Phys* hph;        // pinned host copies
   int last_k;
-  // optional HIP-event timing of kernel launches
+  StepParams sp;    // pool stepping
+  int span;         // pool: slots [0, span) take part in the launched step
+  int stepping;     // pool: a step is launched and not finished
+  hipEvent_t ev_status;
+  int klast;        // pool: last fixed-point launch of the step in flight
+  std::vector<int> live;
+  std::vector<int> fresh;  // loaded since the last status readback
+  std::vector<long> iters_total;
+  // optional HIP-event timing of kernel launches (single simulation)
   int timing;
   struct Rec {
     int e0, e1, cls, k;
@@ -874,6 +976,10 @@ struct amx_fv_ctx {
   int ev_used;
   double ms[2];
   long launches[2];
+};
+
+struct amx_fv_pool {
+  amx_fv_ctx* c;
 };
 
 namespace {
@@ -916,14 +1022,12 @@ void time_resolve(amx_fv_ctx* c, int K) {
   c->ev_used = 0;
 }
 
-}  // namespace
-
-static int env_int(const char* name, int dflt) {
+int env_int(const char* name, int dflt) {
   const char* s = getenv(name);
   return s ? atoi(s) : dflt;
 }
 
-static amx_status validate(const amx_fv_config* c) {
+amx_status validate(const amx_fv_config* c) {
   if (!c) return fail(AMX_ERR_INVALID, "null config%s");
   if (c->dim != 2 && c->dim != 3) return fail(AMX_ERR_UNSUPPORTED, "dimensionality must be 2 or 3%s");
   if (c->nx < 2 || c->ny < 2 || (c->dim == 3 && c->nz < 2)) return fail(AMX_ERR_INVALID, "grid too small%s");
@@ -934,32 +1038,48 @@ static amx_status validate(const amx_fv_config* c) {
   if (c->max_fixed_point_iterations < 0 || c->max_fixed_point_iterations > 100000)
     return fail(AMX_ERR_INVALID, "max_fixed_point_iterations out of range%s");
   if (!(c->grid_spacing > 0)) return fail(AMX_ERR_INVALID, "grid_spacing must be positive%s");
+  if (!(c->gamma > 1.0f)) return fail(AMX_ERR_INVALID, "gamma must exceed 1%s");
   return AMX_OK;
 }
 
-extern "C" amx_status amx_fv_workspace_bytes(const amx_fv_config* cfg, size_t* bytes) {
-  amx_status s = validate(cfg);
-  if (s != AMX_OK) return s;
-  if (!bytes) return fail(AMX_ERR_INVALID, "null output%s");
-  const long n = (long)cfg->nx * cfg->ny * (cfg->dim == 3 ? cfg->nz : 1);
-  const int nv = cfg->dim + 2;
-  *bytes = sizeof(float) * (size_t)n * (nv + (cfg->mhd ? 12 : 0)) + sizeof(Ctl);
-  return AMX_OK;
+long ws_fields(const amx_fv_config* cfg) { return (cfg->dim + 2) + (cfg->mhd ? 12 : 0); }
+long state_fields(const amx_fv_config* cfg) { return (cfg->dim + 2) + (cfg->mhd ? 3 : 0); }
+long cells(const amx_fv_config* cfg) { return (long)cfg->nx * cfg->ny * (cfg->dim == 3 ? cfg->nz : 1); }
+
+Phys make_phys(float gamma, double grid_spacing) {
+  Phys ph;
+  ph.gamma = gamma;
+  ph.gm1 = gamma - 1.0f;
+  ph.inv_gm1 = 1.0f / ph.gm1;
+  ph.inv_dx = (float)(1.0 / grid_spacing);
+  return ph;
 }
 
-extern "C" amx_status amx_fv_create(const amx_fv_config* cfg, amx_fv_ctx** out) {
+void free_ctx(amx_fv_ctx* c) {
+  if (!c) return;
+  (void)hipDeviceSynchronize();
+  if (c->mem) (void)hipFree(c->mem);
+  if (c->ctl) (void)hipFree(c->ctl);
+  if (c->dph) (void)hipFree(c->dph);
+  if (c->hctl) (void)hipHostFree(c->hctl);
+  if (c->hph) (void)hipHostFree(c->hph);
+  if (c->ev_status) (void)hipEventDestroy(c->ev_status);
+  for (auto e : c->ev) (void)hipEventDestroy(e);
+  delete c;
+}
+
+// Geometry, launch shapes and allocations of a context of cap slots.
+amx_status make_ctx(const amx_fv_config* cfg, int cap, int own_state, amx_fv_ctx** out) {
   amx_status s = validate(cfg);
   if (s != AMX_OK) return s;
   if (!out) return fail(AMX_ERR_INVALID, "null output%s");
+  if (cap < 1 || cap > 65535) return fail(AMX_ERR_INVALID, "capacity must be in 1..65535%s");
   amx_fv_ctx* c = new (std::nothrow) amx_fv_ctx();
   if (!c) return fail(AMX_ERR_NOMEM, "host allocation failed%s");
   c->cfg = *cfg;
   c->NV = cfg->dim + 2;
-  c->ncell = (long)cfg->nx * cfg->ny * (cfg->dim == 3 ? cfg->nz : 1);
-  c->ph.gamma = cfg->gamma;
-  c->ph.gm1 = cfg->gamma - 1.0f;
-  c->ph.inv_gm1 = 1.0f / c->ph.gm1;
-  c->ph.inv_dx = (float)(1.0 / cfg->grid_spacing);
+  c->NS = (int)state_fields(cfg);
+  c->ncell = cells(cfg);
   c->dx = (float)cfg->grid_spacing;
   if (cfg->dim == 3) {
     c->nxm = cfg->nx;
@@ -994,37 +1114,368 @@ extern "C" amx_status amx_fv_create(const amx_fv_config* cfg, amx_fv_ctx** out) 
   }
   c->ncx = (c->nxm + c->lx - 1) / c->lx;
   c->ncx_mag = (c->nxm + c->lx_mag - 1) / c->lx_mag;
-  size_t bytes = sizeof(float) * (size_t)c->ncell * (c->NV + (cfg->mhd ? 12 : 0));
-  if (hipMalloc((void**)&c->ws, bytes) != hipSuccess) {
-    delete c;
+  c->cap = cap;
+  c->own_state = own_state;
+  const long per_slot = c->ncell * (ws_fields(cfg) + (own_state ? c->NS : 0));
+  c->sstride = own_state ? per_slot : 0;
+  if (hipMalloc((void**)&c->mem, sizeof(float) * (size_t)per_slot * cap) != hipSuccess) {
+    c->mem = nullptr;
+    free_ctx(c);
     return fail(AMX_ERR_NOMEM, "device workspace allocation failed%s");
   }
-  if (hipMalloc((void**)&c->ctl, sizeof(Ctl)) != hipSuccess) {
-    (void)hipFree(c->ws);
-    delete c;
-    return fail(AMX_ERR_NOMEM, "device control allocation failed%s");
+  if (hipMalloc((void**)&c->ctl, sizeof(Ctl) * cap) != hipSuccess ||
+      hipMalloc((void**)&c->dph, sizeof(Phys) * cap) != hipSuccess ||
+      hipHostMalloc((void**)&c->hctl, sizeof(Ctl) * cap, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&c->hph, sizeof(Phys) * cap, hipHostMallocDefault) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_status, hipEventDisableTiming) != hipSuccess) {
+    free_ctx(c);
+    return fail(AMX_ERR_NOMEM, "control allocation failed%s");
   }
-  if (hipHostMalloc((void**)&c->hctl, sizeof(Ctl), hipHostMallocDefault) != hipSuccess) {
-    (void)hipFree(c->ws);
-    (void)hipFree(c->ctl);
-    delete c;
-    return fail(AMX_ERR_NOMEM, "pinned host allocation failed%s");
+  for (int i = 0; i < cap; ++i) c->hph[i] = make_phys(cfg->gamma, cfg->grid_spacing);
+  if (hipMemcpy(c->dph, c->hph, sizeof(Phys) * cap, hipMemcpyHostToDevice) != hipSuccess) {
+    free_ctx(c);
+    return fail(AMX_ERR_HIP, "parameter upload failed%s");
   }
-  hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, 0, c->ctl);
-  if (hipDeviceSynchronize() != hipSuccess) return fail(AMX_ERR_HIP, "init sync failed%s");
+  // single simulation: slot 0 is live from the start; pool slots start empty
+  hipLaunchKernelGGL(k_slot_reset, dim3(cap), dim3(1), 0, 0, c->ctl, 0, 0.f, own_state ? 0 : 1);
+  if (hipDeviceSynchronize() != hipSuccess) {
+    free_ctx(c);
+    return fail(AMX_ERR_HIP, "init sync failed%s");
+  }
+  c->live.assign(cap, own_state ? 0 : 1);
+  c->fresh.assign(cap, 0);
+  c->iters_total.assign(cap, 0);
   c->last_k = 4;
   *out = c;
   return AMX_OK;
 }
 
+// Slot-0 addresses of the state rows and the workspace.
+struct Bufs {
+  float* gas[5];
+  float* B[3];
+  float* G1;
+  float* P[2][6];
+};
+
+Bufs bufs(const amx_fv_ctx* c, float* state) {
+  Bufs b;
+  memset(&b, 0, sizeof(b));
+  const long n = c->ncell;
+  if (c->own_state) state = c->mem;
+  for (int v = 0; v < c->NV; ++v) b.gas[v] = state + (long)v * n;
+  if (c->cfg.mhd)
+    for (int k = 0; k < 3; ++k) b.B[k] = state + (long)(c->NV + k) * n;
+  float* ws = c->own_state ? c->mem + (long)c->NS * n : c->mem;
+  b.G1 = ws;
+  for (int j = 0; j < 2; ++j)
+    for (int k = 0; k < 6; ++k) b.P[j][k] = ws + (long)(c->NV + 6 * j + k) * n;
+  return b;
+}
+
+template <int DIM, int SRC, bool HEUN, int W0M>
+hipError_t launch_stage_t(const amx_fv_ctx* c, const StageArgs& a, int nspan, hipStream_t st) {
+  const dim3 grid(c->ntb * c->nta * c->ncx, nspan), block((TA_DEF + 1) * TB);
+  if (c->cfg.riemann == AMX_HLLC)
+    hipLaunchKernelGGL((k_stage<DIM, TA_DEF, SRC, HEUN, W0M, AMX_HLLC>), grid, block, 0, st, a);
+  else
+    hipLaunchKernelGGL((k_stage<DIM, TA_DEF, SRC, HEUN, W0M, AMX_HLL>), grid, block, 0, st, a);
+  return hipGetLastError();
+}
+
+template <int SRC, bool HEUN, int W0M>
+hipError_t launch_stage(const amx_fv_ctx* c, const StageArgs& a, int nspan, hipStream_t st) {
+  if (c->cfg.dim == 3) return launch_stage_t<3, SRC, HEUN, W0M>(c, a, nspan, st);
+  return launch_stage_t<2, SRC, HEUN, W0M>(c, a, nspan, st);
+}
+
+StageArgs stage_base(const amx_fv_ctx* c, float dt_scale) {
+  StageArgs a;
+  memset(&a, 0, sizeof(a));
+  a.ctl = c->ctl;
+  a.phs = c->dph;
+  a.sstride = c->sstride;
+  a.slot0 = 0;
+  a.nx = c->nxm;
+  a.na = c->na;
+  a.nb = c->nb;
+  a.ntb = c->ntb;
+  a.nta = c->nta;
+  a.ncx = c->ncx;
+  a.lx = c->lx;
+  a.dt_scale = dt_scale;
+  return a;
+}
+
+MagArgs mag_base(const amx_fv_ctx* c) {
+  MagArgs m;
+  memset(&m, 0, sizeof(m));
+  m.ctl = c->ctl;
+  m.sstride = c->sstride;
+  m.slot0 = 0;
+  m.max_iter = c->cfg.max_fixed_point_iterations;
+  m.tol = c->cfg.fixed_point_tol;
+  m.euler = c->cfg.magnetic_integrator == AMX_IMPLICIT_EULER;
+  m.nx = c->nxm;
+  m.na = c->na;
+  m.nb = c->nb;
+  m.ntb = c->ntb;
+  m.nta = c->nta_mag;
+  m.ncx = c->ncx_mag;
+  m.lx = c->lx_mag;
+  m.inv_dx = (float)(1.0 / c->cfg.grid_spacing);
+  return m;
+}
+
+hipError_t launch_mag(const amx_fv_ctx* c, const MagArgs& m, int nspan, hipStream_t st) {
+  const dim3 grid(c->ntb * c->nta_mag * c->ncx_mag, nspan), block(TA_MAG * TB);
+  if (c->cfg.dim == 3)
+    hipLaunchKernelGGL((k_mag<3, TA_MAG>), grid, block, 0, st, m);
+  else
+    hipLaunchKernelGGL((k_mag<2, TA_MAG>), grid, block, 0, st, m);
+  return hipGetLastError();
+}
+
+// CFL maxima of slots [slot0, slot0 + nspan) of the state whose slot-0 rows start at `state`
+hipError_t launch_cfl(const amx_fv_ctx* c, const float* state, int slot0, int nspan, hipStream_t st) {
+  const long n = c->ncell;
+  CflArgs a;
+  memset(&a, 0, sizeof(a));
+  for (int v = 0; v <= c->cfg.dim; ++v) a.w[v] = state + (long)v * n;
+  a.p = state + (long)(c->NV - 1) * n;
+  a.ctl = c->ctl;
+  a.phs = c->dph;
+  a.sstride = c->sstride;
+  a.n = n;
+  a.slot0 = slot0;
+  long blocks = (n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  if (c->cfg.dim == 3)
+    hipLaunchKernelGGL(k_cfl<3>, dim3(blocks, nspan), dim3(256), 0, st, a);
+  else
+    hipLaunchKernelGGL(k_cfl<2>, dim3(blocks, nspan), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+// ---- one step of slots [0, nspan), dt already in ctl[s].dt, ctl[s].run set.
+// Phase 1: S_A(dt/2) and the speculative batch of fixed-point launches.
+amx_status step_phase1(amx_fv_ctx* c, const Bufs& b, int nspan, hipStream_t st, int want_cfl, int* klast) {
+  const int NV = c->NV;
+  if (!c->cfg.mhd) {
+    // SSP-RK2 on the whole state: W1 = E(W0); W = prim(0.5 (U0 + cons(E(W1)))) in place
+    StageArgs a1 = stage_base(c, 1.0f);
+    for (int v = 0; v < NV; ++v) {
+      a1.src[v] = b.gas[v];
+      a1.out[v] = b.G1 + (long)v * c->ncell;
+    }
+    int e = time_begin(c, st);
+    HIPCHK((launch_stage<SRC_PLAIN, false, SRC_PLAIN>(c, a1, nspan, st)));
+    time_end(c, st, e, CLS_STAGE, 0);
+    StageArgs a2 = stage_base(c, 1.0f);
+    for (int v = 0; v < NV; ++v) {
+      a2.src[v] = b.G1 + (long)v * c->ncell;
+      a2.w0[v] = b.gas[v];
+      a2.out[v] = b.gas[v];
+    }
+    a2.want_cfl = want_cfl;
+    e = time_begin(c, st);
+    HIPCHK((launch_stage<SRC_PLAIN, true, SRC_PLAIN>(c, a2, nspan, st)));
+    time_end(c, st, e, CLS_STAGE, 0);
+    *klast = 0;
+    return AMX_OK;
+  }
+  const int dim = c->cfg.dim;
+  // ---- S_A(dt/2): RK2 with the result in place in the gas rows
+  StageArgs a1 = stage_base(c, 0.5f);
+  for (int v = 0; v < NV; ++v) {
+    a1.src[v] = b.gas[v];
+    a1.out[v] = b.G1 + (long)v * c->ncell;
+  }
+  int e = time_begin(c, st);
+  HIPCHK((launch_stage<SRC_PLAIN, false, SRC_PLAIN>(c, a1, nspan, st)));
+  time_end(c, st, e, CLS_STAGE, 0);
+  StageArgs a2 = stage_base(c, 0.5f);
+  for (int v = 0; v < NV; ++v) {
+    a2.src[v] = b.G1 + (long)v * c->ncell;
+    a2.w0[v] = b.gas[v];
+    a2.out[v] = b.gas[v];
+  }
+  e = time_begin(c, st);
+  HIPCHK((launch_stage<SRC_PLAIN, true, SRC_PLAIN>(c, a2, nspan, st)));
+  time_end(c, st, e, CLS_STAGE, 0);
+  // ---- S_B(dt): fixed point on (B, v) with rho from the gas
+  MagArgs m = mag_base(c);
+  for (int k = 0; k < 3; ++k) {
+    m.B0[k] = b.B[k];
+    m.Bk[k] = b.B[k];
+  }
+  m.v0[0] = b.gas[1];
+  m.v0[1] = b.gas[2];
+  m.v0[2] = dim == 3 ? b.gas[3] : nullptr;
+  m.vk[0] = m.v0[0];
+  m.vk[1] = m.v0[1];
+  m.vk[2] = m.v0[2];
+  m.rho = b.gas[0];
+  for (int k = 0; k < 3; ++k) {
+    m.Bo[k] = b.P[0][k];
+    m.vo[k] = b.P[0][3 + k];
+  }
+  m.k = 0;
+  e = time_begin(c, st);
+  HIPCHK(launch_mag(c, m, nspan, st));
+  time_end(c, st, e, CLS_MAG, 0);
+  int batch = c->last_k + 1;
+  if (batch < 1) batch = 1;
+  const int maxit = c->cfg.max_fixed_point_iterations;
+  *klast = batch < maxit ? batch : maxit;
+  for (int k = 1; k <= *klast; ++k) {
+    MagArgs mk = m;
+    mk.k = k;
+    for (int i = 0; i < 3; ++i) {
+      mk.Bk[i] = b.P[(k - 1) & 1][i];
+      mk.vk[i] = b.P[(k - 1) & 1][3 + i];
+      mk.Bo[i] = b.P[k & 1][i];
+      mk.vo[i] = b.P[k & 1][3 + i];
+    }
+    const int ek = time_begin(c, st);
+    HIPCHK(launch_mag(c, mk, nspan, st));
+    time_end(c, st, ek, CLS_MAG, k);
+  }
+  return AMX_OK;
+}
+
+// Fixed-point state of the slots after a readback of ctl[0, nspan) into hctl:
+// returns 1 when every running slot has converged (or hit the cap) by klast.
+int fixed_point_settled(const amx_fv_ctx* c, int nspan, int klast, int* kmax) {
+  const int maxit = c->cfg.max_fixed_point_iterations;
+  int settled = 1;
+  *kmax = 0;
+  for (int s = 0; s < nspan; ++s) {
+    const Ctl& h = c->hctl[s];
+    if (!h.run) continue;
+    const int K = h.iters;
+    if (K > *kmax) *kmax = K;
+    if (K < klast || K >= maxit) continue;  // a later launch exited: converged; or capped
+    unsigned rb = 0;
+    for (int i = 0; i < NSLOT; ++i) rb = h.resp[K % 3][i] > rb ? h.resp[K % 3][i] : rb;
+    float r;
+    memcpy(&r, &rb, sizeof(float));
+    if (r > c->cfg.fixed_point_tol) settled = 0;
+  }
+  return settled;
+}
+
+// Launches fixed-point iterations klast+1, klast+2, ... (two per round) with a
+// readback after each round until every running slot has settled.
+amx_status fixed_point_more(amx_fv_ctx* c, const Bufs& b, int nspan, hipStream_t st, int* klast) {
+  const int maxit = c->cfg.max_fixed_point_iterations;
+  MagArgs m = mag_base(c);
+  const int dim = c->cfg.dim;
+  for (int k = 0; k < 3; ++k) m.B0[k] = b.B[k];
+  m.v0[0] = b.gas[1];
+  m.v0[1] = b.gas[2];
+  m.v0[2] = dim == 3 ? b.gas[3] : nullptr;
+  m.rho = b.gas[0];
+  int kmax = 0;
+  while (!fixed_point_settled(c, nspan, *klast, &kmax) && *klast < maxit) {
+    const int k1 = *klast + 1, k2 = *klast + 2 < maxit ? *klast + 2 : maxit;
+    for (int k = k1; k <= k2; ++k) {
+      MagArgs mk = m;
+      mk.k = k;
+      for (int i = 0; i < 3; ++i) {
+        mk.Bk[i] = b.P[(k - 1) & 1][i];
+        mk.vk[i] = b.P[(k - 1) & 1][3 + i];
+        mk.Bo[i] = b.P[k & 1][i];
+        mk.vo[i] = b.P[k & 1][3 + i];
+      }
+      const int ek = time_begin(c, st);
+      HIPCHK(launch_mag(c, mk, nspan, st));
+      time_end(c, st, ek, CLS_MAG, k);
+    }
+    *klast = k2;
+    HIPCHK(hipMemcpyAsync(c->hctl, c->ctl, sizeof(Ctl) * nspan, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  return AMX_OK;
+}
+
+// Phase 3: S_A(dt/2) on the magnetically updated gas (v <- v_K, energy fix on
+// the fly); the final iterate of every slot is picked on the device.
+amx_status step_phase3(amx_fv_ctx* c, const Bufs& b, int nspan, hipStream_t st, int want_cfl) {
+  if (!c->cfg.mhd) return AMX_OK;
+  const int NV = c->NV;
+  StageArgs b1 = stage_base(c, 0.5f);
+  for (int v = 0; v < NV; ++v) {
+    b1.src[v] = b.gas[v];
+    b1.out[v] = b.G1 + (long)v * c->ncell;
+  }
+  for (int j = 0; j < 2; ++j)
+    for (int k = 0; k < 3; ++k) {
+      b1.selv[j][k] = b.P[j][3 + k];
+      b1.selb[j][k] = b.P[j][k];
+    }
+  int e = time_begin(c, st);
+  HIPCHK((launch_stage<SRC_MAGFIX, false, SRC_PLAIN>(c, b1, nspan, st)));
+  time_end(c, st, e, CLS_STAGE, 0);
+  StageArgs b2 = stage_base(c, 0.5f);
+  for (int v = 0; v < NV; ++v) {
+    b2.src[v] = b.G1 + (long)v * c->ncell;
+    b2.w0[v] = b.gas[v];
+    b2.out[v] = b.gas[v];
+  }
+  for (int j = 0; j < 2; ++j)
+    for (int k = 0; k < 3; ++k) {
+      b2.selv[j][k] = b.P[j][3 + k];
+      b2.selb[j][k] = b.P[j][k];
+    }
+  for (int k = 0; k < 3; ++k) b2.bdst[k] = b.B[k];
+  b2.copy_b = 1;
+  b2.want_cfl = want_cfl;
+  e = time_begin(c, st);
+  HIPCHK((launch_stage<SRC_PLAIN, true, SRC_MAGFIX>(c, b2, nspan, st)));
+  time_end(c, st, e, CLS_STAGE, 0);
+  return AMX_OK;
+}
+
+// One step of a single simulation (slot 0). Blocks once for the fixed-point
+// readback; hctl[0] then holds dt and t of the step (t before the update).
+amx_status do_step(amx_fv_ctx* c, float* state, hipStream_t st, int want_cfl, int* k_out) {
+  const Bufs b = bufs(c, state);
+  int klast = 0;
+  amx_status s = step_phase1(c, b, 1, st, want_cfl, &klast);
+  if (s != AMX_OK) return s;
+  int K = 0;
+  if (c->cfg.mhd) {
+    HIPCHK(hipMemcpyAsync(c->hctl, c->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    s = fixed_point_more(c, b, 1, st, &klast);
+    if (s != AMX_OK) return s;
+    K = c->hctl[0].iters;
+    time_resolve(c, K);
+    c->last_k = K;
+  }
+  s = step_phase3(c, b, 1, st, want_cfl);
+  if (s != AMX_OK) return s;
+  if (k_out) *k_out = K;
+  return AMX_OK;
+}
+
+}  // namespace
+
+extern "C" amx_status amx_fv_workspace_bytes(const amx_fv_config* cfg, size_t* bytes) {
+  amx_status s = validate(cfg);
+  if (s != AMX_OK) return s;
+  if (!bytes) return fail(AMX_ERR_INVALID, "null output%s");
+  *bytes = sizeof(float) * (size_t)cells(cfg) * ws_fields(cfg) + sizeof(Ctl) + sizeof(Phys);
+  return AMX_OK;
+}
+
+extern "C" amx_status amx_fv_create(const amx_fv_config* cfg, amx_fv_ctx** out) {
+  return make_ctx(cfg, 1, 0, out);
+}
+
 extern "C" amx_status amx_fv_destroy(amx_fv_ctx* c) {
-  if (!c) return AMX_OK;
-  (void)hipDeviceSynchronize();
-  (void)hipFree(c->ws);
-  (void)hipFree(c->ctl);
-  (void)hipHostFree(c->hctl);
-  for (auto e : c->ev) (void)hipEventDestroy(e);
-  delete c;
+  free_ctx(c);
   return AMX_OK;
 }
 
@@ -1050,234 +1501,25 @@ extern "C" amx_status amx_fv_get_timing(amx_fv_ctx* c, amx_fv_timing* out) {
   return AMX_OK;
 }
 
-namespace {
-
-template <int DIM, int SRC, bool HEUN, int W0M>
-hipError_t launch_stage_t(const amx_fv_ctx* c, const StageArgs& a, hipStream_t st) {
-  const dim3 grid(c->ntb * c->nta * c->ncx), block((TA_DEF + 1) * TB);
-  if (c->cfg.riemann == AMX_HLLC)
-    hipLaunchKernelGGL((k_stage<DIM, TA_DEF, SRC, HEUN, W0M, AMX_HLLC>), grid, block, 0, st, a);
-  else
-    hipLaunchKernelGGL((k_stage<DIM, TA_DEF, SRC, HEUN, W0M, AMX_HLL>), grid, block, 0, st, a);
-  return hipGetLastError();
-}
-
-template <int SRC, bool HEUN, int W0M>
-hipError_t launch_stage(const amx_fv_ctx* c, const StageArgs& a, hipStream_t st) {
-  if (c->cfg.dim == 3) return launch_stage_t<3, SRC, HEUN, W0M>(c, a, st);
-  return launch_stage_t<2, SRC, HEUN, W0M>(c, a, st);
-}
-
-StageArgs stage_base(const amx_fv_ctx* c, const float* dtp, float dt_scale) {
-  StageArgs a;
-  memset(&a, 0, sizeof(a));
-  a.dtp = dtp;
-  a.nx = c->nxm;
-  a.na = c->na;
-  a.nb = c->nb;
-  a.ntb = c->ntb;
-  a.nta = c->nta;
-  a.ncx = c->ncx;
-  a.lx = c->lx;
-  a.dt_scale = dt_scale;
-  a.ph = c->ph;
-  return a;
-}
-
-MagArgs mag_base(const amx_fv_ctx* c, const float* dtp) {
-  MagArgs m;
-  memset(&m, 0, sizeof(m));
-  m.dtp = dtp;
-  m.ctl = c->ctl;
-  m.max_iter = c->cfg.max_fixed_point_iterations;
-  m.tol = c->cfg.fixed_point_tol;
-  m.euler = c->cfg.magnetic_integrator == AMX_IMPLICIT_EULER;
-  m.nx = c->nxm;
-  m.na = c->na;
-  m.nb = c->nb;
-  m.ntb = c->ntb;
-  m.nta = c->nta_mag;
-  m.ncx = c->ncx_mag;
-  m.lx = c->lx_mag;
-  m.inv_dx = c->ph.inv_dx;
-  return m;
-}
-
-hipError_t launch_mag(const amx_fv_ctx* c, const MagArgs& m, hipStream_t st) {
-  const dim3 grid(c->ntb * c->nta_mag * c->ncx_mag), block(TA_MAG * TB);
-  if (c->cfg.dim == 3)
-    hipLaunchKernelGGL((k_mag<3, TA_MAG>), grid, block, 0, st, m);
-  else
-    hipLaunchKernelGGL((k_mag<2, TA_MAG>), grid, block, 0, st, m);
-  return hipGetLastError();
-}
-
-hipError_t launch_cfl(const amx_fv_ctx* c, const float* state, hipStream_t st) {
-  const long n = c->ncell;
-  const float* f[5];
-  for (int v = 0; v < c->NV; ++v) f[v] = state + (long)v * n;
-  long blocks = (n + 255) / 256;
-  if (blocks > 4096) blocks = 4096;
-  unsigned* out = &c->ctl->cflp[0][0];
-  if (c->cfg.dim == 3)
-    hipLaunchKernelGGL(k_cfl<3>, dim3(blocks), dim3(256), 0, st, f[0], f[1], f[2], f[3], f[4], n, c->cfg.gamma, out);
-  else
-    hipLaunchKernelGGL(k_cfl<2>, dim3(blocks), dim3(256), 0, st, f[0], f[1], f[2], (const float*)nullptr, f[3], n,
-                       c->cfg.gamma, out);
-  return hipGetLastError();
-}
-
-// One step on `state` with dt already in ctl->dt. Returns the fixed-point count.
-amx_status do_step(amx_fv_ctx* c, float* state, hipStream_t st, int want_cfl, int* k_out) {
-  const long n = c->ncell;
-  const int NV = c->NV;
-  const float* dtp = &c->ctl->dt;
-  float* G1 = c->ws;
-  float* gas[5];
-  for (int v = 0; v < NV; ++v) gas[v] = state + (long)v * n;
-  if (!c->cfg.mhd) {
-    // SSP-RK2 on the whole state: W1 = E(W0); W = prim(0.5 (U0 + cons(E(W1)))) in place
-    StageArgs a1 = stage_base(c, dtp, 1.0f);
-    for (int v = 0; v < NV; ++v) {
-      a1.src[v] = gas[v];
-      a1.out[v] = G1 + (long)v * n;
-    }
-    int e = time_begin(c, st);
-    HIPCHK((launch_stage<SRC_PLAIN, false, SRC_PLAIN>(c, a1, st)));
-    time_end(c, st, e, CLS_STAGE, 0);
-    StageArgs a2 = stage_base(c, dtp, 1.0f);
-    for (int v = 0; v < NV; ++v) {
-      a2.src[v] = G1 + (long)v * n;
-      a2.w0[v] = gas[v];
-      a2.out[v] = gas[v];
-    }
-    a2.cflmax = want_cfl ? &c->ctl->cflp[0][0] : nullptr;
-    e = time_begin(c, st);
-    HIPCHK((launch_stage<SRC_PLAIN, true, SRC_PLAIN>(c, a2, st)));
-    time_end(c, st, e, CLS_STAGE, 0);
-    if (k_out) *k_out = 0;
-    return AMX_OK;
-  }
-  float* Bs[3];
-  for (int k = 0; k < 3; ++k) Bs[k] = state + (long)(NV + k) * n;
-  float* P[2][6];
-  for (int j = 0; j < 2; ++j)
-    for (int k = 0; k < 6; ++k) P[j][k] = c->ws + (long)(NV + 6 * j + k) * n;
-  const int dim = c->cfg.dim;
-  // ---- S_A(dt/2): RK2 with the result in place in the gas rows
-  StageArgs a1 = stage_base(c, dtp, 0.5f);
-  for (int v = 0; v < NV; ++v) {
-    a1.src[v] = gas[v];
-    a1.out[v] = G1 + (long)v * n;
-  }
-  int e = time_begin(c, st);
-  HIPCHK((launch_stage<SRC_PLAIN, false, SRC_PLAIN>(c, a1, st)));
-  time_end(c, st, e, CLS_STAGE, 0);
-  StageArgs a2 = stage_base(c, dtp, 0.5f);
-  for (int v = 0; v < NV; ++v) {
-    a2.src[v] = G1 + (long)v * n;
-    a2.w0[v] = gas[v];
-    a2.out[v] = gas[v];
-  }
-  e = time_begin(c, st);
-  HIPCHK((launch_stage<SRC_PLAIN, true, SRC_PLAIN>(c, a2, st)));
-  time_end(c, st, e, CLS_STAGE, 0);
-  // ---- S_B(dt): fixed point on (B, v) with rho from the gas
-  MagArgs m = mag_base(c, dtp);
-  for (int k = 0; k < 3; ++k) {
-    m.B0[k] = Bs[k];
-    m.Bk[k] = Bs[k];
-  }
-  m.v0[0] = gas[1];
-  m.v0[1] = gas[2];
-  m.v0[2] = dim == 3 ? gas[3] : nullptr;
-  m.vk[0] = m.v0[0];
-  m.vk[1] = m.v0[1];
-  m.vk[2] = m.v0[2];
-  m.rho = gas[0];
-  for (int k = 0; k < 3; ++k) {
-    m.Bo[k] = P[0][k];
-    m.vo[k] = P[0][3 + k];
-  }
-  m.k = 0;
-  e = time_begin(c, st);
-  HIPCHK(launch_mag(c, m, st));
-  time_end(c, st, e, CLS_MAG, 0);
-  const int maxit = c->cfg.max_fixed_point_iterations;
-  int kfirst = 1;
-  int batch = c->last_k + 1;
-  if (batch < 1) batch = 1;
-  int K = 0;
-  for (;;) {
-    const int klast = kfirst + batch - 1 < maxit ? kfirst + batch - 1 : maxit;
-    for (int k = kfirst; k <= klast; ++k) {
-      MagArgs mk = m;
-      mk.k = k;
-      for (int i = 0; i < 3; ++i) {
-        mk.Bk[i] = P[(k - 1) & 1][i];
-        mk.vk[i] = P[(k - 1) & 1][3 + i];
-        mk.Bo[i] = P[k & 1][i];
-        mk.vo[i] = P[k & 1][3 + i];
-      }
-      const int ek = time_begin(c, st);
-      HIPCHK(launch_mag(c, mk, st));
-      time_end(c, st, ek, CLS_MAG, k);
-    }
-    HIPCHK(hipMemcpyAsync(c->hctl, c->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    K = c->hctl->iters;
-    time_resolve(c, K);
-    unsigned rb = 0;
-    for (int i = 0; i < NSLOT; ++i) rb = c->hctl->resp[K % 3][i] > rb ? c->hctl->resp[K % 3][i] : rb;
-    float r;
-    memcpy(&r, &rb, sizeof(float));
-    const bool done = !(r > c->cfg.fixed_point_tol) || K >= maxit;
-    if (done || klast >= maxit) break;
-    if (K < klast) break;  // a kernel exited early: converged (defensive)
-    kfirst = klast + 1;
-    batch = 2;
-  }
-  c->last_k = K;
-  // ---- S_A(dt/2) on the magnetically updated gas (v <- vn, energy fix on the fly)
-  float* const* Pn = P[K & 1];
-  StageArgs b1 = stage_base(c, dtp, 0.5f);
-  for (int v = 0; v < NV; ++v) b1.src[v] = gas[v];
-  b1.src[5] = Pn[3];
-  b1.src[6] = Pn[4];
-  b1.src[7] = Pn[5];
-  for (int v = 0; v < NV; ++v) b1.out[v] = G1 + (long)v * n;
-  e = time_begin(c, st);
-  HIPCHK((launch_stage<SRC_MAGFIX, false, SRC_PLAIN>(c, b1, st)));
-  time_end(c, st, e, CLS_STAGE, 0);
-  StageArgs b2 = stage_base(c, dtp, 0.5f);
-  for (int v = 0; v < NV; ++v) {
-    b2.src[v] = G1 + (long)v * n;
-    b2.w0[v] = gas[v];
-    b2.out[v] = gas[v];
-  }
-  b2.w0[5] = Pn[3];
-  b2.w0[6] = Pn[4];
-  b2.w0[7] = Pn[5];
-  for (int k = 0; k < 3; ++k) {
-    b2.bsrc[k] = Pn[k];
-    b2.bdst[k] = Bs[k];
-  }
-  b2.cflmax = want_cfl ? &c->ctl->cflp[0][0] : nullptr;
-  e = time_begin(c, st);
-  HIPCHK((launch_stage<SRC_PLAIN, true, SRC_MAGFIX>(c, b2, st)));
-  time_end(c, st, e, CLS_STAGE, 0);
-  if (k_out) *k_out = K;
+extern "C" amx_status amx_fv_set_physics(amx_fv_ctx* c, float gamma, float C_cfl) {
+  if (!c) return fail(AMX_ERR_INVALID, "null context%s");
+  if (c->own_state) return fail(AMX_ERR_INVALID, "pool slots take gamma at amx_fv_pool_load%s");
+  if (!(gamma > 1.0f) || !(C_cfl > 0.0f)) return fail(AMX_ERR_INVALID, "need gamma > 1 and C_cfl > 0%s");
+  HIPCHK(hipDeviceSynchronize());  // work in flight still reads the old values
+  c->cfg.gamma = gamma;
+  c->cfg.C_cfl = C_cfl;
+  c->hph[0] = make_phys(gamma, c->cfg.grid_spacing);
+  HIPCHK(hipMemcpy(c->dph, c->hph, sizeof(Phys), hipMemcpyHostToDevice));
   return AMX_OK;
 }
 
-}  // namespace
-
 extern "C" amx_status amx_fv_cfl_dt(amx_fv_ctx* c, const float* state, float* d_dt, void* stream) {
   if (!c || !state || !d_dt) return fail(AMX_ERR_INVALID, "null argument%s");
+  if (c->own_state) return fail(AMX_ERR_INVALID, "a pool context is stepped with amx_fv_pool_*%s");
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(k_zero_cfl, dim3(1), dim3(1), 0, st, c->ctl);
   HIPCHK(hipGetLastError());
-  HIPCHK(launch_cfl(c, state, st));
+  HIPCHK(launch_cfl(c, state, 0, 1, st));
   hipLaunchKernelGGL(k_cfl_out, dim3(1), dim3(1), 0, st, c->ctl, c->cfg.C_cfl, c->dx, c->cfg.dim, d_dt);
   HIPCHK(hipGetLastError());
   return AMX_OK;
@@ -1286,6 +1528,7 @@ extern "C" amx_status amx_fv_cfl_dt(amx_fv_ctx* c, const float* state, float* d_
 extern "C" amx_status amx_fv_evolve_state(amx_fv_ctx* c, float* state, const float* d_dt, int* iterations,
                                           void* stream) {
   if (!c || !state || !d_dt) return fail(AMX_ERR_INVALID, "null argument%s");
+  if (c->own_state) return fail(AMX_ERR_INVALID, "a pool context is stepped with amx_fv_pool_*%s");
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, st, c->ctl, d_dt);
   HIPCHK(hipGetLastError());
@@ -1299,22 +1542,19 @@ extern "C" amx_status amx_fv_evolve_state(amx_fv_ctx* c, float* state, const flo
 extern "C" amx_status amx_fv_time_integration(amx_fv_ctx* c, float* state, const amx_fv_run* run,
                                               amx_fv_stats* stats, void* stream) {
   if (!c || !state || !run) return fail(AMX_ERR_INVALID, "null argument%s");
+  if (c->own_state) return fail(AMX_ERR_INVALID, "a pool context is stepped with amx_fv_pool_*%s");
   if (run->fixed_timestep && run->num_timesteps <= 0) return fail(AMX_ERR_INVALID, "num_timesteps must be positive%s");
   hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, st, c->ctl);
+  hipLaunchKernelGGL(k_slot_reset, dim3(1), dim3(1), 0, st, c->ctl, 0, run->t_end, 1);
   HIPCHK(hipGetLastError());
-  if (!run->fixed_timestep) {
-    HIPCHK(launch_cfl(c, state, st));
-  }
+  if (!run->fixed_timestep) HIPCHK(launch_cfl(c, state, 0, 1, st));
   StepParams sp;
-  sp.t_end = run->t_end;
   sp.fixed = run->fixed_timestep;
   sp.num_timesteps = run->num_timesteps;
   sp.exact_end = run->exact_end_time;
   sp.C = c->cfg.C_cfl;
   sp.dx = c->dx;
   sp.dim = c->cfg.dim;
-  sp.use_cfl = 1;
   long steps = 0, iters = 0;
   int kmax = 0;
   float t = 0.f;
@@ -1337,10 +1577,10 @@ extern "C" amx_status amx_fv_time_integration(amx_fv_ctx* c, float* state, const
       HIPCHK(hipMemcpyAsync(c->hctl, c->ctl, offsetof(Ctl, cflp), hipMemcpyDeviceToHost, st));
       HIPCHK(hipStreamSynchronize(st));
       time_resolve(c, 1 << 30);
-      t = c->hctl->t;
+      t = c->hctl[0].t;
     } else {
       // hctl holds dt and t from the fixed-point readback (before k_step_end)
-      t = c->hctl->t + c->hctl->dt;
+      t = c->hctl[0].t + c->hctl[0].dt;
     }
     iters += K;
     if (K > kmax) kmax = K;
@@ -1351,10 +1591,176 @@ extern "C" amx_status amx_fv_time_integration(amx_fv_ctx* c, float* state, const
   time_resolve(c, 1 << 30);
   if (stats) {
     stats->steps = steps;
-    stats->t = c->hctl->t;
+    stats->t = c->hctl[0].t;
     stats->fixed_point_iterations = iters;
     stats->max_fixed_point_iterations = kmax;
   }
+  return AMX_OK;
+}
+
+// ============================================================================
+// Pools: continuous batching of independent simulations of one grid
+// ============================================================================
+
+extern "C" amx_status amx_fv_pool_slot_bytes(const amx_fv_config* cfg, size_t* bytes) {
+  amx_status s = validate(cfg);
+  if (s != AMX_OK) return s;
+  if (!bytes) return fail(AMX_ERR_INVALID, "null output%s");
+  *bytes = sizeof(float) * (size_t)cells(cfg) * (ws_fields(cfg) + state_fields(cfg)) + sizeof(Ctl) + sizeof(Phys);
+  return AMX_OK;
+}
+
+extern "C" amx_status amx_fv_pool_create(const amx_fv_config* cfg, const amx_fv_run* run, int capacity,
+                                         amx_fv_pool** out) {
+  if (!out || !run) return fail(AMX_ERR_INVALID, "null argument%s");
+  if (run->fixed_timestep && run->num_timesteps <= 0) return fail(AMX_ERR_INVALID, "num_timesteps must be positive%s");
+  amx_fv_ctx* c = nullptr;
+  amx_status s = make_ctx(cfg, capacity, 1, &c);
+  if (s != AMX_OK) return s;
+  c->sp.fixed = run->fixed_timestep;
+  c->sp.num_timesteps = run->num_timesteps;
+  c->sp.exact_end = run->exact_end_time;
+  c->sp.C = cfg->C_cfl;
+  c->sp.dx = c->dx;
+  c->sp.dim = cfg->dim;
+  amx_fv_pool* p = new (std::nothrow) amx_fv_pool();
+  if (!p) {
+    free_ctx(c);
+    return fail(AMX_ERR_NOMEM, "host allocation failed%s");
+  }
+  p->c = c;
+  *out = p;
+  return AMX_OK;
+}
+
+extern "C" amx_status amx_fv_pool_destroy(amx_fv_pool* p) {
+  if (!p) return AMX_OK;
+  free_ctx(p->c);
+  delete p;
+  return AMX_OK;
+}
+
+static amx_status pool_slot_check(amx_fv_pool* p, int slot) {
+  if (!p || !p->c) return fail(AMX_ERR_INVALID, "null pool%s");
+  if (slot < 0 || slot >= p->c->cap) return fail(AMX_ERR_INVALID, "slot out of range%s");
+  if (p->c->stepping) return fail(AMX_ERR_INVALID, "a step is in flight: call amx_fv_pool_step_finish first%s");
+  return AMX_OK;
+}
+
+extern "C" amx_status amx_fv_pool_load(amx_fv_pool* p, int slot, const float* state, float gamma, float t_end,
+                                       void* stream) {
+  amx_status s = pool_slot_check(p, slot);
+  if (s != AMX_OK) return s;
+  if (!state) return fail(AMX_ERR_INVALID, "null state%s");
+  if (!(gamma > 1.0f)) return fail(AMX_ERR_INVALID, "gamma must exceed 1%s");
+  if (!(t_end >= 0.0f)) return fail(AMX_ERR_INVALID, "t_end must be >= 0%s");
+  amx_fv_ctx* c = p->c;
+  hipStream_t st = (hipStream_t)stream;
+  float* dst = c->mem + (long)slot * c->sstride;
+  HIPCHK(hipMemcpyAsync(dst, state, sizeof(float) * (size_t)c->NS * c->ncell, hipMemcpyDefault, st));
+  HIPCHK(hipStreamSynchronize(st));  // the pinned parameter mirror is rewritten below
+  c->hph[slot] = make_phys(gamma, c->cfg.grid_spacing);
+  HIPCHK(hipMemcpyAsync(c->dph + slot, c->hph + slot, sizeof(Phys), hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_slot_reset, dim3(1), dim3(1), 0, st, c->ctl, slot, t_end, 1);
+  HIPCHK(hipGetLastError());
+  if (!c->sp.fixed) HIPCHK(launch_cfl(c, c->mem, slot, 1, st));
+  c->live[slot] = 1;
+  c->fresh[slot] = 1;
+  c->iters_total[slot] = 0;
+  return AMX_OK;
+}
+
+extern "C" amx_status amx_fv_pool_release(amx_fv_pool* p, int slot, void* stream) {
+  amx_status s = pool_slot_check(p, slot);
+  if (s != AMX_OK) return s;
+  hipLaunchKernelGGL(k_slot_reset, dim3(1), dim3(1), 0, (hipStream_t)stream, p->c->ctl, slot, 0.f, 0);
+  HIPCHK(hipGetLastError());
+  p->c->live[slot] = 0;
+  p->c->fresh[slot] = 0;
+  return AMX_OK;
+}
+
+extern "C" amx_status amx_fv_pool_read(amx_fv_pool* p, int slot, float* dst, void* stream) {
+  amx_status s = pool_slot_check(p, slot);
+  if (s != AMX_OK) return s;
+  if (!dst) return fail(AMX_ERR_INVALID, "null destination%s");
+  amx_fv_ctx* c = p->c;
+  HIPCHK(hipMemcpyAsync(dst, c->mem + (long)slot * c->sstride, sizeof(float) * (size_t)c->NS * c->ncell,
+                        hipMemcpyDefault, (hipStream_t)stream));
+  return AMX_OK;
+}
+
+extern "C" amx_status amx_fv_pool_step_launch(amx_fv_pool* p, void* stream) {
+  if (!p || !p->c) return fail(AMX_ERR_INVALID, "null pool%s");
+  amx_fv_ctx* c = p->c;
+  if (c->stepping) return fail(AMX_ERR_INVALID, "a step is already in flight%s");
+  int span = 0;
+  for (int s = 0; s < c->cap; ++s)
+    if (c->live[s]) span = s + 1;
+  c->span = span;
+  if (span == 0) return AMX_OK;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_step_begin, dim3(span), dim3(1), 0, st, c->ctl, c->sp);
+  HIPCHK(hipGetLastError());
+  const Bufs b = bufs(c, nullptr);
+  int klast = 0;
+  amx_status s = step_phase1(c, b, span, st, !c->sp.fixed, &klast);
+  if (s != AMX_OK) return s;
+  c->klast = klast;  // the launched batch, read back by step_finish
+  HIPCHK(hipMemcpyAsync(c->hctl, c->ctl, sizeof(Ctl) * span, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipEventRecord(c->ev_status, st));
+  c->stepping = 1;
+  return AMX_OK;
+}
+
+extern "C" amx_status amx_fv_pool_step_finish(amx_fv_pool* p, void* stream) {
+  if (!p || !p->c) return fail(AMX_ERR_INVALID, "null pool%s");
+  amx_fv_ctx* c = p->c;
+  if (!c->stepping) return AMX_OK;
+  c->stepping = 0;
+  const int span = c->span;
+  hipStream_t st = (hipStream_t)stream;
+  HIPCHK(hipEventSynchronize(c->ev_status));
+  const Bufs b = bufs(c, nullptr);
+  int klast = c->klast;
+  int kmax = 0;
+  if (c->cfg.mhd) {
+    amx_status s = fixed_point_more(c, b, span, st, &klast);
+    if (s != AMX_OK) return s;
+    fixed_point_settled(c, span, klast, &kmax);
+    c->last_k = kmax;
+    for (int i = 0; i < span; ++i)
+      if (c->hctl[i].run) c->iters_total[i] += c->hctl[i].iters;
+  }
+  amx_status s = step_phase3(c, b, span, st, !c->sp.fixed);
+  if (s != AMX_OK) return s;
+  hipLaunchKernelGGL(k_step_end, dim3(span), dim3(1), 0, st, c->ctl);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(c->hctl, c->ctl, sizeof(Ctl) * span, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipEventRecord(c->ev_status, st));
+  for (int i = 0; i < span; ++i) c->fresh[i] = 0;
+  return AMX_OK;
+}
+
+extern "C" amx_status amx_fv_pool_status(amx_fv_pool* p, int slot, amx_fv_slot_status* out) {
+  if (!p || !p->c || !out) return fail(AMX_ERR_INVALID, "null argument%s");
+  amx_fv_ctx* c = p->c;
+  if (slot < 0 || slot >= c->cap) return fail(AMX_ERR_INVALID, "slot out of range%s");
+  if (c->stepping) return fail(AMX_ERR_INVALID, "a step is in flight: call amx_fv_pool_step_finish first%s");
+  HIPCHK(hipEventSynchronize(c->ev_status));
+  memset(out, 0, sizeof(*out));
+  out->live = c->live[slot];
+  if (!out->live) return AMX_OK;
+  if (c->fresh[slot] || slot >= c->span) {
+    // loaded after the last step: nothing stepped yet
+    return AMX_OK;
+  }
+  const Ctl& h = c->hctl[slot];
+  out->steps = h.steps;
+  out->t = h.t;
+  out->t_end = h.t_end;
+  out->fixed_point_iterations = c->iters_total[slot];
+  out->finished = c->sp.fixed ? (h.steps >= c->sp.num_timesteps) : !(h.t < h.t_end);
   return AMX_OK;
 }
 

@@ -82,6 +82,11 @@ amx_status amx_fv_create(const amx_fv_config* cfg, amx_fv_ctx** out);
 /* Frees the context and its workspace. */
 amx_status amx_fv_destroy(amx_fv_ctx* ctx);
 
+/* Changes gamma and C_cfl of a context (params.gamma / params.C_cfl are
+ * traced, not static, in the reference: time_integration.py:108-140), so one
+ * context serves simulations that differ only in them. Synchronises the device. */
+amx_status amx_fv_set_physics(amx_fv_ctx* ctx, float gamma, float C_cfl);
+
 /* Bytes of device workspace a context of this configuration allocates. */
 amx_status amx_fv_workspace_bytes(const amx_fv_config* cfg, size_t* bytes);
 
@@ -120,6 +125,51 @@ typedef struct amx_fv_timing {
  * loop, time_integration.py:194-234); used by the benchmark harness. */
 amx_status amx_fv_set_timing(amx_fv_ctx* ctx, int enable);
 amx_status amx_fv_get_timing(amx_fv_ctx* ctx, amx_fv_timing* out);
+
+/* ---------------------------------------------------------------------------
+ * Pools: continuous batching of independent finite-volume simulations of one
+ * grid and scheme. Every launch of a pool step covers all its running slots
+ * (the slot is a grid dimension of each kernel); each slot has its own state,
+ * gamma, t_end, dt, CFL reduction and fixed-point iteration count, so its
+ * result equals the single-simulation path's on the same input. Slots are
+ * loaded and released between steps while the others keep going. No
+ * counterpart in the reference (one simulation per time_integration call,
+ * time_integration.py:61-236); used by the serving engine.
+ * ------------------------------------------------------------------------- */
+typedef struct amx_fv_pool amx_fv_pool;
+
+typedef struct amx_fv_slot_status {
+  int live;                       /* the slot holds a simulation */
+  int finished;                   /* t reached t_end (or num_timesteps steps) */
+  long steps;
+  double t;
+  double t_end;
+  long fixed_point_iterations;    /* sum over the slot's steps */
+} amx_fv_slot_status;
+
+/* Device bytes one slot of this configuration takes (state + workspace). */
+amx_status amx_fv_pool_slot_bytes(const amx_fv_config* cfg, size_t* bytes);
+/* A pool of `capacity` empty slots on the current device; `run` gives the
+ * step-size rule (adaptive CFL or fixed, exact end); its t_end is unused. */
+amx_status amx_fv_pool_create(const amx_fv_config* cfg, const amx_fv_run* run, int capacity,
+                              amx_fv_pool** out);
+amx_status amx_fv_pool_destroy(amx_fv_pool* pool);
+/* Copies a primitive state (host or device memory, reference layout) into a
+ * slot and starts it at t = 0 with its own gamma and t_end. */
+amx_status amx_fv_pool_load(amx_fv_pool* pool, int slot, const float* state, float gamma, float t_end,
+                            void* stream);
+/* Copies a slot's state out (host or device memory). */
+amx_status amx_fv_pool_read(amx_fv_pool* pool, int slot, float* dst, void* stream);
+/* Empties a slot. */
+amx_status amx_fv_pool_release(amx_fv_pool* pool, int slot, void* stream);
+/* One step of every running slot, in two halves so that several pools (on
+ * their own streams) overlap: _launch enqueues the step up to the fixed
+ * point's first readback and returns; _finish waits for it, runs any further
+ * fixed-point iterations, enqueues the rest of the step and returns. */
+amx_status amx_fv_pool_step_launch(amx_fv_pool* pool, void* stream);
+amx_status amx_fv_pool_step_finish(amx_fv_pool* pool, void* stream);
+/* Status of a slot after the last finished step (waits for its readback). */
+amx_status amx_fv_pool_status(amx_fv_pool* pool, int slot, amx_fv_slot_status* out);
 
 /* ---------------------------------------------------------------------------
  * Finite-difference MHD (config.solver_mode == FINITE_DIFFERENCE): 3D only,

@@ -121,6 +121,7 @@ def test_native_struct_layout_matches_header():
     assert ctypes.sizeof(_native.AmxFvRun) == 24
     assert ctypes.sizeof(_native.AmxFvStats) == 32
     assert ctypes.sizeof(_native.AmxFdConfig) == 48
+    assert ctypes.sizeof(_native.AmxFvSlotStatus) == 40
 
 
 def test_no_gpu_means_loud_failure(monkeypatch):

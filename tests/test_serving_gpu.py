@@ -1,0 +1,135 @@
+"""Continuous batching (amx_fv_pool_* and the serving engine) on the GPU.
+
+Every slot of a pool runs the same kernels as the single-simulation path
+with its own state, gamma, t_end, dt and fixed-point count, so each result
+must equal the single-simulation result for the same request bit for bit,
+whether it ran alone or in a mix, admitted at the start or mid-run.
+"""
+
+import numpy as np
+import pytest
+
+from astronomix_amd import _native
+from astronomix_amd import workloads as W
+from astronomix_amd.serving import Request, ServingEngine, initial_state, mixed_requests
+from astronomix_amd.serving.engine import _pool_config, run_one_at_a_time
+
+pytestmark = pytest.mark.gpu
+
+
+def solo(req, state):
+    import torch
+    ctx = _native.fv_context(_pool_config(req.kind, req.n, req.gamma))
+    s = torch.from_numpy(np.ascontiguousarray(state, dtype=np.float32)).cuda()
+    run = _native.AmxFvRun(t_end=float(req.t_end), fixed_timestep=0, num_timesteps=0, exact_end_time=1,
+                           max_steps=-1)
+    st = ctx.time_integration(s, run)
+    torch.cuda.synchronize()
+    return s.cpu().numpy(), st
+
+
+def step_until_done(pool, slots):
+    done = {}
+    while len(done) < len(slots):
+        pool.step_launch()
+        pool.step_finish()
+        for slot in slots:
+            if slot not in done and pool.status(slot).finished:
+                done[slot] = pool.status(slot)
+    return done
+
+
+def test_pool_slots_equal_single_path_2d(gpu):
+    import torch
+    print("[gpu] pool 2D OT 64^2 x 3", flush=True)
+    reqs = [Request(0, "orszag_tang_2d", 64, 1.4, 0.05, 0, 0), Request(1, "orszag_tang_2d", 64, 5 / 3, 0.08, 0, 0),
+            Request(2, "orszag_tang_2d", 64, 1.55, 0.03, 0, 0)]
+    pool = _native.FvPool(_pool_config("orszag_tang_2d", 64), 4)
+    state = W.orszag_tang_2d(64)
+    for i, r in enumerate(reqs):
+        pool.load(i, state, r.gamma, r.t_end)
+    st = step_until_done(pool, [0, 1, 2])
+    for i, r in enumerate(reqs):
+        out = pool.read(i, torch.empty(state.shape, dtype=torch.float32, device="cuda")).cpu().numpy()
+        ref, rst = solo(r, state)
+        assert st[i].steps == rst.steps
+        assert abs(st[i].t - r.t_end) < 1e-6
+        np.testing.assert_array_equal(out, ref)
+    pool.close()
+
+
+def test_pool_mid_run_admission_3d(gpu):
+    """Capacity 2, three blasts: the third is loaded into the slot the first
+    frees while the second keeps running."""
+    import torch
+    print("[gpu] pool 3D blast 24^3 mid-run admission", flush=True)
+    reqs = [Request(0, "mhd_blast_3d", 24, 1.6, 0.002, 20.0, 0), Request(1, "mhd_blast_3d", 24, 1.45, 0.006, 8.0, 0),
+            Request(2, "mhd_blast_3d", 24, 5 / 3, 0.004, 40.0, 0)]
+    states = {r.rid: initial_state(r) for r in reqs}
+    pool = _native.FvPool(_pool_config("mhd_blast_3d", 24), 2)
+    pool.load(0, states[0], reqs[0].gamma, reqs[0].t_end)
+    pool.load(1, states[1], reqs[1].gamma, reqs[1].t_end)
+    owner = {0: 0, 1: 1}
+    results = {}
+    steps_taken = 0
+    while len(results) < 3:
+        pool.step_launch()
+        pool.step_finish()
+        steps_taken += 1
+        for slot in (0, 1):
+            rid = owner.get(slot)
+            if rid is None:
+                continue
+            st = pool.status(slot)
+            if st.finished:
+                results[rid] = pool.read(slot, torch.empty(states[rid].shape, dtype=torch.float32,
+                                                           device="cuda")).cpu().numpy()
+                pool.release(slot)
+                owner[slot] = None
+                if 2 not in owner.values() and 2 not in results:
+                    pool.load(slot, states[2], reqs[2].gamma, reqs[2].t_end)
+                    owner[slot] = 2
+        assert steps_taken < 10000
+    for r in reqs:
+        ref, _ = solo(r, states[r.rid])
+        np.testing.assert_array_equal(results[r.rid], ref)
+    pool.close()
+
+
+def test_engine_mix_equals_alone(gpu):
+    """A small mixed stream through the engine (2D and 3D pools overlapping
+    on their own streams) against each request served alone."""
+    print("[gpu] engine mixed stream", flush=True)
+    grids = (("orszag_tang_2d", 64), ("orszag_tang_2d", 96), ("mhd_blast_3d", 24), ("mhd_blast_3d", 32))
+    reqs = mixed_requests(10, rate=1000.0, seed=3, grids=grids, t_end_range=(0.002, 0.02))
+    states = {r.rid: initial_state(r) for r in reqs}
+    eng = ServingEngine(hbm_budget=2 << 30, slots_per_pool=3)
+    res = eng.serve(reqs, states, realtime=False)
+    eng.close()
+    assert sorted(res) == [r.rid for r in reqs]
+    alone = run_one_at_a_time(reqs, states, realtime=False)
+    for r in reqs:
+        np.testing.assert_array_equal(res[r.rid]["state"].cpu().numpy(), alone[r.rid]["state"].cpu().numpy())
+        assert res[r.rid]["steps"] == alone[r.rid]["steps"]
+
+
+def test_engine_admission_respects_hbm_budget(gpu):
+    """A budget of one 32^3 slot: requests queue, run one at a time, all finish."""
+    print("[gpu] engine budget", flush=True)
+    cfg = _pool_config("mhd_blast_3d", 32)
+    slot = _native.pool_slot_bytes(cfg)
+    reqs = mixed_requests(3, rate=1000.0, seed=5, grids=(("mhd_blast_3d", 32),), t_end_range=(0.002, 0.004))
+    eng = ServingEngine(hbm_budget=slot + slot // 2, slots_per_pool=8)
+    peak = 0
+    orig = eng.step
+
+    def step():
+        nonlocal peak
+        peak = max(peak, eng.running())
+        assert eng.used <= eng.budget
+        return orig()
+
+    eng.step = step
+    res = eng.serve(reqs, realtime=False)
+    eng.close()
+    assert len(res) == 3 and peak == 1
