@@ -28,7 +28,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <cstdint>
+#include <map>
 #include <new>
+#include <utility>
 #include <vector>
 
 #include "amx.h"
@@ -963,6 +965,9 @@ struct amx_fv_ctx {
   int stepping;     // pool: a step is launched and not finished
   hipEvent_t ev_status;
   int klast;        // pool: last fixed-point launch of the step in flight
+  int use_graphs;   // pool: replay captured HIP graphs of the step halves
+  std::map<std::pair<int, int>, hipGraphExec_t> g_launch;  // (span, klast)
+  std::map<int, hipGraphExec_t> g_finish;                  // span
   std::vector<int> live;
   std::vector<int> fresh;  // loaded since the last status readback
   std::vector<long> iters_total;
@@ -1064,6 +1069,8 @@ void free_ctx(amx_fv_ctx* c) {
   if (c->hctl) (void)hipHostFree(c->hctl);
   if (c->hph) (void)hipHostFree(c->hph);
   if (c->ev_status) (void)hipEventDestroy(c->ev_status);
+  for (auto& g : c->g_launch) (void)hipGraphExecDestroy(g.second);
+  for (auto& g : c->g_finish) (void)hipGraphExecDestroy(g.second);
   for (auto e : c->ev) (void)hipEventDestroy(e);
   delete c;
 }
@@ -1602,6 +1609,36 @@ extern "C" amx_status amx_fv_time_integration(amx_fv_ctx* c, float* state, const
 // Pools: continuous batching of independent simulations of one grid
 // ============================================================================
 
+namespace {
+
+// Runs `enqueue` on `st`: directly, or -- with graphs on -- by replaying the
+// graph cached under `slot` (captured from `enqueue` on first use). The
+// enqueued work is kernels only (no memset nodes).
+template <class F>
+amx_status replay_or_capture(amx_fv_ctx* c, hipStream_t st, hipGraphExec_t* slot, bool have, F&& enqueue) {
+  if (!c->use_graphs) return enqueue();
+  if (!have) {
+    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    const amx_status s = enqueue();
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(st, &g);
+    if (s != AMX_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return s;
+    }
+    HIPCHK(e);
+    hipGraphExec_t x = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIPCHK(ei);
+    *slot = x;
+  }
+  HIPCHK(hipGraphLaunch(*slot, st));
+  return AMX_OK;
+}
+
+}  // namespace
+
 extern "C" amx_status amx_fv_pool_slot_bytes(const amx_fv_config* cfg, size_t* bytes) {
   amx_status s = validate(cfg);
   if (s != AMX_OK) return s;
@@ -1623,6 +1660,7 @@ extern "C" amx_status amx_fv_pool_create(const amx_fv_config* cfg, const amx_fv_
   c->sp.C = cfg->C_cfl;
   c->sp.dx = c->dx;
   c->sp.dim = cfg->dim;
+  c->use_graphs = env_int("AMX_POOL_GRAPHS", 1) != 0;
   amx_fv_pool* p = new (std::nothrow) amx_fv_pool();
   if (!p) {
     free_ctx(c);
@@ -1700,12 +1738,26 @@ extern "C" amx_status amx_fv_pool_step_launch(amx_fv_pool* p, void* stream) {
   c->span = span;
   if (span == 0) return AMX_OK;
   hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_step_begin, dim3(span), dim3(1), 0, st, c->ctl, c->sp);
-  HIPCHK(hipGetLastError());
   const Bufs b = bufs(c, nullptr);
   int klast = 0;
-  amx_status s = step_phase1(c, b, span, st, !c->sp.fixed, &klast);
+  if (c->cfg.mhd) {
+    // the speculative fixed-point batch of step_phase1
+    const int maxit = c->cfg.max_fixed_point_iterations;
+    klast = c->last_k + 1 < 1 ? 1 : c->last_k + 1;
+    if (klast > maxit) klast = maxit;
+    c->last_k = klast - 1;
+  }
+  const auto key = std::make_pair(span, klast);
+  auto it = c->g_launch.find(key);
+  hipGraphExec_t x = it == c->g_launch.end() ? nullptr : it->second;
+  amx_status s = replay_or_capture(c, st, &x, x != nullptr, [&]() -> amx_status {
+    hipLaunchKernelGGL(k_step_begin, dim3(span), dim3(1), 0, st, c->ctl, c->sp);
+    HIPCHK(hipGetLastError());
+    int kl = 0;
+    return step_phase1(c, b, span, st, !c->sp.fixed, &kl);
+  });
   if (s != AMX_OK) return s;
+  if (x) c->g_launch[key] = x;
   c->klast = klast;  // the launched batch, read back by step_finish
   HIPCHK(hipMemcpyAsync(c->hctl, c->ctl, sizeof(Ctl) * span, hipMemcpyDeviceToHost, st));
   HIPCHK(hipEventRecord(c->ev_status, st));
@@ -1732,13 +1784,27 @@ extern "C" amx_status amx_fv_pool_step_finish(amx_fv_pool* p, void* stream) {
     for (int i = 0; i < span; ++i)
       if (c->hctl[i].run) c->iters_total[i] += c->hctl[i].iters;
   }
-  amx_status s = step_phase3(c, b, span, st, !c->sp.fixed);
+  auto it = c->g_finish.find(span);
+  hipGraphExec_t x = it == c->g_finish.end() ? nullptr : it->second;
+  amx_status s = replay_or_capture(c, st, &x, x != nullptr, [&]() -> amx_status {
+    amx_status r = step_phase3(c, b, span, st, !c->sp.fixed);
+    if (r != AMX_OK) return r;
+    hipLaunchKernelGGL(k_step_end, dim3(span), dim3(1), 0, st, c->ctl);
+    HIPCHK(hipGetLastError());
+    return AMX_OK;
+  });
   if (s != AMX_OK) return s;
-  hipLaunchKernelGGL(k_step_end, dim3(span), dim3(1), 0, st, c->ctl);
-  HIPCHK(hipGetLastError());
+  if (x) c->g_finish[span] = x;
   HIPCHK(hipMemcpyAsync(c->hctl, c->ctl, sizeof(Ctl) * span, hipMemcpyDeviceToHost, st));
   HIPCHK(hipEventRecord(c->ev_status, st));
   for (int i = 0; i < span; ++i) c->fresh[i] = 0;
+  return AMX_OK;
+}
+
+extern "C" amx_status amx_fv_pool_set_graphs(amx_fv_pool* p, int enable) {
+  if (!p || !p->c) return fail(AMX_ERR_INVALID, "null pool%s");
+  if (p->c->stepping) return fail(AMX_ERR_INVALID, "a step is in flight%s");
+  p->c->use_graphs = enable ? 1 : 0;
   return AMX_OK;
 }
 

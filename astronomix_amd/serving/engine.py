@@ -61,6 +61,12 @@ class ServingEngine:
                 self.used -= self.pool_bytes.pop(cls)
                 del self.pools[cls], self.slot_req[cls]
 
+    def reserve(self, classes):
+        """Creates the pools of these grid classes ahead of the stream."""
+        for cls in classes:
+            if cls not in self.pools:
+                self._make_pool(cls)
+
     def try_admit(self, req: Request, state) -> bool:
         cls = req.grid_class
         pool = self.pools.get(cls)

@@ -178,6 +178,67 @@ def bench_fd(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_serving(args):
+    """Continuous batching of the mixed request stream (Poisson 50/s, seed 0:
+    Orszag-Tang 64^2..512^2 and blast 64^3..128^3, gamma ~ U[1.4, 5/3], blast
+    p_in ~ U[5, 50], t_end ~ U[0.1, 0.5]) against the same stream served one
+    simulation at a time; both keep states on the GPU and hand back GPU
+    tensors. Pools and contexts are created before the clock starts."""
+    import numpy as np
+    import torch
+    from astronomix_amd import _native
+    from astronomix_amd.serving import ServingEngine, initial_state, latency_summary, mixed_requests
+    from astronomix_amd.serving.engine import _pool_config, run_one_at_a_time
+    from astronomix_amd.serving.requests import GRIDS
+
+    reqs = mixed_requests(args.requests, rate=50.0, seed=0)
+    states = {r.rid: initial_state(r) for r in reqs}
+    eng = ServingEngine(hbm_budget=int(args.hbm_budget_gib * (1 << 30)), slots_per_pool=args.slots)
+    eng.reserve(sorted({r.grid_class for r in reqs}))
+    for kind, n in GRIDS:
+        _native.fv_context(_pool_config(kind, n))
+    torch.cuda.synchronize()
+    served = eng.serve(reqs, states, realtime=True)
+    torch.cuda.synchronize()
+    used = eng.used
+    eng.close()
+    alone = run_one_at_a_time(reqs, states, realtime=True)
+    diff = max(float((served[r.rid]["state"] - alone[r.rid]["state"]).abs().max()) for r in reqs)
+    s_served, s_alone = latency_summary(served), latency_summary(alone)
+    zone_updates = sum(served[r.rid]["steps"] * int(np.prod(states[r.rid].shape[1:])) for r in reqs)
+    out = {
+        "metric": "served solves/s (continuous batching) and p99 latency, mixed OT/blast stream",
+        "value": s_served["solves_per_s"],
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": int(sum(served[r.rid]["steps"] for r in reqs)),
+        "warmup": 0,
+        "ms_per_step": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"serving_mix: {len(reqs)} requests, Poisson 50/s seed 0; Orszag-Tang 64^2-512^2 (HLL, CFL 0.4) "
+                        f"and MHD blast 64^3-128^3 (HLL, CFL 0.3), gamma~U[1.4,5/3], p_in~U[5,50], t_end~U[0.1,0.5]",
+            "slots_per_pool": args.slots,
+            "hbm_budget_gib": args.hbm_budget_gib,
+            "parallelism": "single GPU",
+        },
+        "serving": s_served,
+        "one_at_a_time": s_alone,
+        "speedup_solves_per_s": round(s_served["solves_per_s"] / s_alone["solves_per_s"], 3),
+        "p99_ratio_alone_over_served": round(s_alone["p99_latency_s"] / s_served["p99_latency_s"], 3),
+        "max_abs_diff_served_vs_alone": diff,
+        "zone_updates_per_s_served": round(zone_updates / s_served["makespan_s"] / 1e6, 2),
+        "pool_bytes": used,
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def stream_copy_gbs(gib: float = 1.0, reps: int = 10):
     """HBM ceiling on this box: the library's streaming copy kernel on a
     buffer of `gib` GiB (read + write counted), timed with HIP events."""
@@ -209,7 +270,11 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=None,
                     help="CPU baseline sample length (default: 8 FV steps at 256^3, 6 FD steps at 128^3)")
     ap.add_argument("--copy-gib", type=float, default=1.0)
-    ap.add_argument("--workload", default="mhd_blast_3d", choices=["mhd_blast_3d", "mhd_turbulence_3d_fd"])
+    ap.add_argument("--workload", default="mhd_blast_3d",
+                    choices=["mhd_blast_3d", "mhd_turbulence_3d_fd", "serving_mix"])
+    ap.add_argument("--requests", type=int, default=32, help="serving_mix: requests in the stream")
+    ap.add_argument("--slots", type=int, default=16, help="serving_mix: slots per grid-class pool")
+    ap.add_argument("--hbm-budget-gib", type=float, default=32.0, help="serving_mix: pool memory budget")
     ap.add_argument("--no-kernel-timing", action="store_true",
                     help="time the steps without per-launch HIP events (no roofline figures)")
     args = ap.parse_args()
@@ -220,6 +285,8 @@ def main():
 
     if args.workload == "mhd_turbulence_3d_fd":
         return bench_fd(args)
+    if args.workload == "serving_mix":
+        return bench_serving(args)
 
     import torch
     from astronomix_amd import _native

@@ -168,6 +168,10 @@ amx_status amx_fv_pool_release(amx_fv_pool* pool, int slot, void* stream);
  * fixed-point iterations, enqueues the rest of the step and returns. */
 amx_status amx_fv_pool_step_launch(amx_fv_pool* pool, void* stream);
 amx_status amx_fv_pool_step_finish(amx_fv_pool* pool, void* stream);
+/* Pool steps replay HIP graphs captured per (running-slot span, fixed-point
+ * batch) by default (env AMX_POOL_GRAPHS=0 turns this off at creation);
+ * enable = 0 launches the same kernels directly. */
+amx_status amx_fv_pool_set_graphs(amx_fv_pool* pool, int enable);
 /* Status of a slot after the last finished step (waits for its readback). */
 amx_status amx_fv_pool_status(amx_fv_pool* pool, int slot, amx_fv_slot_status* out);
 
