@@ -103,6 +103,7 @@ SIGNATURES = {
     "amx_fv_set_timing": (ctypes.c_int, [_P, ctypes.c_int]),
     "amx_fv_get_timing": (ctypes.c_int, [_P, ctypes.POINTER(AmxFvTiming)]),
     "amx_fv_set_physics": (ctypes.c_int, [_P, ctypes.c_float, ctypes.c_float]),
+    "amx_fv_adjoint_step": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
     "amx_fv_pool_slot_bytes": (ctypes.c_int, [ctypes.POINTER(AmxFvConfig), ctypes.POINTER(ctypes.c_size_t)]),
     "amx_fv_pool_create": (ctypes.c_int, [ctypes.POINTER(AmxFvConfig), ctypes.POINTER(AmxFvRun), ctypes.c_int,
                                           ctypes.POINTER(_P)]),
@@ -213,6 +214,11 @@ class FvContext:
         t = AmxFvTiming()
         check(self.lib.amx_fv_get_timing(self.handle, ctypes.byref(t)), "amx_fv_get_timing")
         return t
+
+    def adjoint_step(self, state, dt, lam_out, lam_in, stream=None):
+        check(self.lib.amx_fv_adjoint_step(self.handle, ctypes.c_void_p(state.data_ptr()), ctypes.c_void_p(dt.data_ptr()),
+                                           ctypes.c_void_p(lam_out.data_ptr()), ctypes.c_void_p(lam_in.data_ptr()),
+                                           self._stream(stream)), "amx_fv_adjoint_step")
 
     def set_physics(self, gamma: float, C_cfl: float):
         check(self.lib.amx_fv_set_physics(self.handle, float(gamma), float(C_cfl)), "amx_fv_set_physics")

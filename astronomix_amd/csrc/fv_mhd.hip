@@ -907,6 +907,8 @@ __global__ void k_copy_tail(float* __restrict__ dst, const float* __restrict__ s
 
 }  // namespace amx
 
+#include "fv_adjoint.inc"
+
 // ============================================================================
 // C ABI
 // ============================================================================
@@ -964,6 +966,7 @@ struct amx_fv_ctx {
   int span;         // pool: slots [0, span) take part in the launched step
   int stepping;     // pool: a step is launched and not finished
   hipEvent_t ev_status;
+  float* adj;       // reverse-mode workspace (allocated on first use)
   int klast;        // pool: last fixed-point launch of the step in flight
   int use_graphs;   // pool: replay captured HIP graphs of the step halves
   std::map<std::pair<int, int>, hipGraphExec_t> g_launch;  // (span, klast)
@@ -1064,6 +1067,7 @@ void free_ctx(amx_fv_ctx* c) {
   if (!c) return;
   (void)hipDeviceSynchronize();
   if (c->mem) (void)hipFree(c->mem);
+  if (c->adj) (void)hipFree(c->adj);
   if (c->ctl) (void)hipFree(c->ctl);
   if (c->dph) (void)hipFree(c->dph);
   if (c->hctl) (void)hipHostFree(c->hctl);
@@ -1603,6 +1607,119 @@ extern "C" amx_status amx_fv_time_integration(amx_fv_ctx* c, float* state, const
     stats->max_fixed_point_iterations = kmax;
   }
   return AMX_OK;
+}
+
+// ============================================================================
+// Reverse mode of one hydro SSP-RK2 step
+// ============================================================================
+
+namespace {
+
+template <int DIM>
+amx_status adjoint_step_t(amx_fv_ctx* c, const float* w0, const float* lam_out, float* lam_in, hipStream_t st) {
+  constexpr int NV = DIM + 2;
+  const long n = c->ncell;
+  adj::AdjGeom g;
+  if (DIM == 3) {
+    g.n[0] = c->cfg.nx;
+    g.n[1] = c->cfg.ny;
+    g.n[2] = c->cfg.nz;
+    g.st[2] = 1;
+    g.st[1] = c->cfg.nz;
+    g.st[0] = (long)c->cfg.ny * c->cfg.nz;
+  } else {
+    g.n[0] = c->cfg.nx;
+    g.n[1] = c->cfg.ny;
+    g.n[2] = 1;
+    g.st[1] = 1;
+    g.st[0] = c->cfg.ny;
+    g.st[2] = n;
+  }
+  g.nc = n;
+  float* G1 = c->mem;                  // W1 = E(W0) (the context's stage buffer)
+  float* G2 = c->adj;                  // W2 = E(W1)
+  float* L2 = G2 + (long)NV * n;       // lam of W2
+  float* NU = L2 + (long)NV * n;
+  float* L1 = NU + (long)NV * n;       // lam of W1
+  float* CON = L1 + (long)NV * n;      // DIM x 4 x NV interface contributions
+  // recompute the two stage states with the forward kernel (dt in ctl[0])
+  StageArgs a1 = stage_base(c, 1.0f);
+  for (int v = 0; v < NV; ++v) {
+    a1.src[v] = w0 + (long)v * n;
+    a1.out[v] = G1 + (long)v * n;
+  }
+  HIPCHK((launch_stage<SRC_PLAIN, false, SRC_PLAIN>(c, a1, 1, st)));
+  StageArgs a2 = stage_base(c, 1.0f);
+  for (int v = 0; v < NV; ++v) {
+    a2.src[v] = G1 + (long)v * n;
+    a2.out[v] = G2 + (long)v * n;
+  }
+  HIPCHK((launch_stage<SRC_PLAIN, false, SRC_PLAIN>(c, a2, 1, st)));
+  // dt from the device control block: the coefficient dt/dx is needed on the
+  // host for the interface kernels (one readback per step)
+  HIPCHK(hipMemcpyAsync(c->hctl, c->ctl, sizeof(float), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  const Phys ph = c->hph[0];
+  const float coef = ph.inv_dx * c->hctl[0].dt;
+  long blocks = (n + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  const dim3 grid(blocks), blk(256);
+  hipLaunchKernelGGL(adj::k_adj_heun<DIM>, grid, blk, 0, st, w0, G2, lam_out, lam_in, L2, g, ph);
+  HIPCHK(hipGetLastError());
+  // two stages in reverse: (W1 -> W2, lam = L2) then (W0 -> W1, lam = L1)
+  for (int stage = 1; stage >= 0; --stage) {
+    const float* Win = stage == 1 ? G1 : w0;
+    const float* Wout = stage == 1 ? G2 : G1;
+    const float* lam = stage == 1 ? L2 : L1;
+    float* dst = stage == 1 ? L1 : lam_in;
+    hipLaunchKernelGGL(adj::k_adj_nu<DIM>, grid, blk, 0, st, Wout, lam, NU, g, ph);
+    HIPCHK(hipGetLastError());
+    for (int ax = 0; ax < DIM; ++ax) {
+      float* con = CON + (long)ax * 4 * NV * n;
+      if (ax == 0) {
+        if (c->cfg.riemann == AMX_HLLC)
+          hipLaunchKernelGGL((adj::k_adj_iface<DIM, 0, AMX_HLLC>), grid, blk, 0, st, Win, NU, con, coef, g, ph);
+        else
+          hipLaunchKernelGGL((adj::k_adj_iface<DIM, 0, AMX_HLL>), grid, blk, 0, st, Win, NU, con, coef, g, ph);
+      } else if (ax == 1) {
+        if (c->cfg.riemann == AMX_HLLC)
+          hipLaunchKernelGGL((adj::k_adj_iface<DIM, 1, AMX_HLLC>), grid, blk, 0, st, Win, NU, con, coef, g, ph);
+        else
+          hipLaunchKernelGGL((adj::k_adj_iface<DIM, 1, AMX_HLL>), grid, blk, 0, st, Win, NU, con, coef, g, ph);
+      } else {
+        if (c->cfg.riemann == AMX_HLLC)
+          hipLaunchKernelGGL((adj::k_adj_iface<DIM, 2, AMX_HLLC>), grid, blk, 0, st, Win, NU, con, coef, g, ph);
+        else
+          hipLaunchKernelGGL((adj::k_adj_iface<DIM, 2, AMX_HLL>), grid, blk, 0, st, Win, NU, con, coef, g, ph);
+      }
+      HIPCHK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(adj::k_adj_gather<DIM>, grid, blk, 0, st, Win, NU, CON, dst, stage == 0 ? 1 : 0, g, ph);
+    HIPCHK(hipGetLastError());
+  }
+  return AMX_OK;
+}
+
+}  // namespace
+
+extern "C" amx_status amx_fv_adjoint_step(amx_fv_ctx* c, const float* state, const float* d_dt, const float* lam_out,
+                                          float* lam_in, void* stream) {
+  if (!c || !state || !d_dt || !lam_out || !lam_in) return fail(AMX_ERR_INVALID, "null argument%s");
+  if (c->own_state) return fail(AMX_ERR_INVALID, "a pool context has no reverse mode%s");
+  if (c->cfg.mhd) return fail(AMX_ERR_UNSUPPORTED, "reverse mode covers the hydro step only%s");
+  if (lam_in == lam_out || lam_in == state) return fail(AMX_ERR_INVALID, "lam_in must not alias its inputs%s");
+  const int NV = c->NV, dim = c->cfg.dim;
+  if (!c->adj) {
+    const size_t fields = (size_t)4 * NV + (size_t)dim * 4 * NV;
+    if (hipMalloc((void**)&c->adj, sizeof(float) * fields * c->ncell) != hipSuccess) {
+      c->adj = nullptr;
+      return fail(AMX_ERR_NOMEM, "reverse-mode workspace allocation failed%s");
+    }
+  }
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, st, c->ctl, d_dt);
+  HIPCHK(hipGetLastError());
+  return dim == 3 ? adjoint_step_t<3>(c, state, lam_out, lam_in, st) : adjoint_step_t<2>(c, state, lam_out, lam_in, st);
 }
 
 // ============================================================================

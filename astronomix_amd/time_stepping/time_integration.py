@@ -32,6 +32,13 @@ def time_integration(primitive_state, config, params, registered_variables, snap
         from astronomix_amd._finite_difference import time_integration_fd
         return time_integration_fd(primitive_state, config, params, registered_variables)
     require_supported(config, registered_variables)
+    if isinstance(primitive_state, torch.Tensor) and primitive_state.requires_grad:
+        # differentiation through the loop (differentiation_mode=BACKWARDS in
+        # the reference): reverse-mode kernels with checkpointed recomputation
+        from astronomix_amd.adjoint import rollout
+        every = max(1, -(-int(config.num_timesteps) // max(1, int(config.num_checkpoints)))) \
+            if config.fixed_timestep else 50
+        return rollout(primitive_state, config, params, registered_variables, checkpoint_every=every)
 
     struct = None
     if config.state_struct:
