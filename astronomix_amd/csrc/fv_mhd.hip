@@ -1655,12 +1655,7 @@ amx_status adjoint_step_t(amx_fv_ctx* c, const float* w0, const float* lam_out, 
     a2.out[v] = G2 + (long)v * n;
   }
   HIPCHK((launch_stage<SRC_PLAIN, false, SRC_PLAIN>(c, a2, 1, st)));
-  // dt from the device control block: the coefficient dt/dx is needed on the
-  // host for the interface kernels (one readback per step)
-  HIPCHK(hipMemcpyAsync(c->hctl, c->ctl, sizeof(float), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
-  const Phys ph = c->hph[0];
-  const float coef = ph.inv_dx * c->hctl[0].dt;
+  const Phys ph = c->hph[0];  // the interface kernels read dt from ctl[0]
   long blocks = (n + 255) / 256;
   if (blocks > 8192) blocks = 8192;
   const dim3 grid(blocks), blk(256);
@@ -1678,19 +1673,19 @@ amx_status adjoint_step_t(amx_fv_ctx* c, const float* w0, const float* lam_out, 
       float* con = CON + (long)ax * 4 * NV * n;
       if (ax == 0) {
         if (c->cfg.riemann == AMX_HLLC)
-          hipLaunchKernelGGL((adj::k_adj_iface<DIM, 0, AMX_HLLC>), grid, blk, 0, st, Win, NU, con, coef, g, ph);
+          hipLaunchKernelGGL((adj::k_adj_iface<DIM, 0, AMX_HLLC>), grid, blk, 0, st, Win, NU, con, c->ctl, g, ph);
         else
-          hipLaunchKernelGGL((adj::k_adj_iface<DIM, 0, AMX_HLL>), grid, blk, 0, st, Win, NU, con, coef, g, ph);
+          hipLaunchKernelGGL((adj::k_adj_iface<DIM, 0, AMX_HLL>), grid, blk, 0, st, Win, NU, con, c->ctl, g, ph);
       } else if (ax == 1) {
         if (c->cfg.riemann == AMX_HLLC)
-          hipLaunchKernelGGL((adj::k_adj_iface<DIM, 1, AMX_HLLC>), grid, blk, 0, st, Win, NU, con, coef, g, ph);
+          hipLaunchKernelGGL((adj::k_adj_iface<DIM, 1, AMX_HLLC>), grid, blk, 0, st, Win, NU, con, c->ctl, g, ph);
         else
-          hipLaunchKernelGGL((adj::k_adj_iface<DIM, 1, AMX_HLL>), grid, blk, 0, st, Win, NU, con, coef, g, ph);
+          hipLaunchKernelGGL((adj::k_adj_iface<DIM, 1, AMX_HLL>), grid, blk, 0, st, Win, NU, con, c->ctl, g, ph);
       } else {
         if (c->cfg.riemann == AMX_HLLC)
-          hipLaunchKernelGGL((adj::k_adj_iface<DIM, 2, AMX_HLLC>), grid, blk, 0, st, Win, NU, con, coef, g, ph);
+          hipLaunchKernelGGL((adj::k_adj_iface<DIM, 2, AMX_HLLC>), grid, blk, 0, st, Win, NU, con, c->ctl, g, ph);
         else
-          hipLaunchKernelGGL((adj::k_adj_iface<DIM, 2, AMX_HLL>), grid, blk, 0, st, Win, NU, con, coef, g, ph);
+          hipLaunchKernelGGL((adj::k_adj_iface<DIM, 2, AMX_HLL>), grid, blk, 0, st, Win, NU, con, c->ctl, g, ph);
       }
       HIPCHK(hipGetLastError());
     }

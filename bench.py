@@ -239,6 +239,78 @@ def bench_serving(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_adjoint(args):
+    """Gradient of L = sum (rho_final - target)^2 with respect to the initial
+    v_y field of the 2D Kelvin-Helmholtz start (512^2, HLLC, minmod, SSP-RK2,
+    1000 fixed steps to t = 0.2, checkpoint every 50 steps): forward run +
+    checkpointed reverse sweep through the adjoint kernels."""
+    import numpy as np
+    import torch
+    from astronomix_amd import workloads as W
+    from astronomix_amd.adjoint import rollout
+    from astronomix_amd.option_classes.simulation_config import HLLC
+    from astronomix_amd.option_classes.simulation_params import SimulationParams
+
+    n, steps = args.n if args.n != 256 else 512, args.steps if args.steps != 200 else 1000
+    cfg = W.fv_config(2, n, mhd=False, riemann=HLLC)._replace(fixed_timestep=True, num_timesteps=steps)
+    params = SimulationParams(C_cfl=0.4, t_end=0.2 * steps / 1000, gamma=5.0 / 3.0)
+    s0 = torch.from_numpy(W.kelvin_helmholtz_2d(n, seed=0)).cuda()
+    target = torch.from_numpy(np.random.default_rng(1).random((n, n)).astype(np.float32)).cuda()
+
+    def grad_vy():
+        vy = s0[2].clone().requires_grad_(True)
+        w = torch.cat([s0[:2], vy[None], s0[3:]])
+        out = rollout(w, cfg, params, checkpoint_every=50)
+        loss = ((out[0].double() - target.double()) ** 2).sum()
+        loss.backward()
+        return loss.detach(), vy.grad
+
+    if args.warmup > 0:
+        grad_vy()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    loss, g = grad_vy()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    # the forward alone, for the cost split
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    with torch.no_grad():
+        rollout(s0.clone(), cfg, params, checkpoint_every=50)
+    torch.cuda.synchronize()
+    fwd = time.perf_counter() - t1
+    out = {
+        "metric": "adjoint: zone-updates/s through forward + checkpointed reverse sweep, 2D KH 512^2 HLLC",
+        "value": round(steps * n * n / el / 1e6, 3),
+        "unit": "Mzone-updates/s (forward+reverse)",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(el / steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"adjoint_kh_{n}: grad of sum((rho_T - target)^2) wrt initial v_y; KH rho=2/1, vx=+-0.5, p=2.5, "
+                        f"vy=0.01 sin(4 pi x) + N(0,1e-3) seed 0; HLLC + minmod, SSP-RK2, {steps} fixed steps, "
+                        f"checkpoint every 50",
+            "grid": [n, n],
+            "parallelism": "single GPU",
+        },
+        "gradient_seconds": round(el, 4),
+        "forward_seconds": round(fwd, 4),
+        "reverse_over_forward": round((el - fwd) / fwd, 3),
+        "loss": float(loss),
+        "grad_norm": float(g.double().norm()),
+        "grad_finite": bool(torch.isfinite(g).all()),
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def stream_copy_gbs(gib: float = 1.0, reps: int = 10):
     """HBM ceiling on this box: the library's streaming copy kernel on a
     buffer of `gib` GiB (read + write counted), timed with HIP events."""
@@ -271,7 +343,7 @@ def main():
                     help="CPU baseline sample length (default: 8 FV steps at 256^3, 6 FD steps at 128^3)")
     ap.add_argument("--copy-gib", type=float, default=1.0)
     ap.add_argument("--workload", default="mhd_blast_3d",
-                    choices=["mhd_blast_3d", "mhd_turbulence_3d_fd", "serving_mix"])
+                    choices=["mhd_blast_3d", "mhd_turbulence_3d_fd", "serving_mix", "adjoint_kh"])
     ap.add_argument("--requests", type=int, default=32, help="serving_mix: requests in the stream")
     ap.add_argument("--slots", type=int, default=16, help="serving_mix: slots per grid-class pool")
     ap.add_argument("--hbm-budget-gib", type=float, default=32.0, help="serving_mix: pool memory budget")
@@ -287,6 +359,8 @@ def main():
         return bench_fd(args)
     if args.workload == "serving_mix":
         return bench_serving(args)
+    if args.workload == "adjoint_kh":
+        return bench_adjoint(args)
 
     import torch
     from astronomix_amd import _native
