@@ -49,6 +49,10 @@ constexpr int TB = 64;      // tile width along the contiguous axis (one wavefro
 // serialise on one address; the consumer reduces the slots. All reduced values
 // are >= 0 floats stored as bits, so unsigned max == float max (NaN, made
 // canonical 0x7fc00000, wins as in jnp.max).
+// residual slot sets: a pair launch (L, L+1) reads L-2, L-1, accumulates
+// into L, L+1 and clears L+2, L+3 for the next pair (all mod NRES)
+constexpr int NRES = 6;
+
 struct Ctl {
   float dt;
   float t;
@@ -60,7 +64,7 @@ struct Ctl {
   float t_end; // end time of the slot's simulation
   unsigned cfl[4];              // reduced CFL maxima per axis
   unsigned cflp[3][NSLOT];      // CFL partial maxima
-  unsigned resp[3][NSLOT];      // fixed-point residual partials, rotating by iteration % 3
+  unsigned resp[NRES][NSLOT];   // fixed-point residual partials, by launch index % NRES
 };
 
 struct Phys {
@@ -275,11 +279,11 @@ struct StageArgs {
   const float* w0[8];    // Heun: stage-0 state at the cell centre
   float* out[5];         // output gas rows
   float* bdst[3];        // optional copy of the final B of the magnetic sub-step
-  // final fixed-point iterate of the slot, chosen on the device by the parity
-  // of its iteration count: v_K replaces src[5..7] (SRC_MAGFIX) and w0[5..7]
+  // final fixed-point iterate of the slot, chosen on the device by its launch
+  // index K mod 3: v_K replaces src[5..7] (SRC_MAGFIX) and w0[5..7]
   // (W0M == SRC_MAGFIX), B_K is the copy source
-  const float* selv[2][3];
-  const float* selb[2][3];
+  const float* selv[3][3];
+  const float* selb[3][3];
   Ctl* ctl;
   const Phys* phs;
   long sstride;
@@ -317,7 +321,7 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
   Ctl* const C = a.ctl + slot;
   if (!C->run) return;  // block-uniform: empty or finished slot
   const long so = slot * a.sstride;
-  const int par = C->iters & 1;
+  const int par = C->iters % 3;  // buffer of the fixed point's final iterate
   const float* src[8];
   const float* w0p[8];
 #pragma unroll
@@ -534,20 +538,32 @@ struct MagArgs {
   const float* B0[3];
   const float* v0[3];  // v0[2] == nullptr in 2D (vz = 0)
   const float* rho;
-  const float* Bk[3];
+  const float* Bk[3];  // input iterate: output of launch L-1 (the state's B, v for L = 0)
   const float* vk[3];  // vk[2] == nullptr: vz = 0
-  float* Bo[3];
-  float* vo[3];
+  float* Bo[2][3];     // outputs of launches L and L+1
+  float* vo[2][3];
   Ctl* ctl;            // per-slot control, slot = slot0 + blockIdx.y
   long sstride;        // floats between consecutive slots' buffers
   int slot0;
-  int k;         // 0: initial iterate (Bk = B0, vk = v0); k >= 1: while body k
+  int L;               // first launch index of the pair (even)
   int max_iter;
   float tol;
-  int euler;     // IMPLICIT_EULER: average = iterate
+  int euler;           // IMPLICIT_EULER: average = iterate
   int nx, na, nb, ntb, nta, ncx, lx;
   float inv_dx;
 };
+
+// The fixed point's stopping rule on an executed pair (L, L+1) with residual
+// maxima r0 = res_L, r1 = res_{L+1} (res_k = max(max|B_k - B_{k-1}|,
+// max|v_k - v_{k-1}|), launch k making iterate k+1): launch k runs while
+// res_{k-1} > tol and k-1 < max_iter (_magnetic_field_update.py:127-141).
+// Returns the last launch that the reference would run (L or L+1), or -1 when
+// launch L+2 would still run.
+__host__ __device__ inline int pair_final(int L, float r0, float r1, float tol, int max_iter) {
+  if (!(r0 > tol) || L >= max_iter) return L;
+  if (!(r1 > tol) || L + 1 >= max_iter) return L + 1;
+  return -1;
+}
 
 // Raw iterate and fixed-point inputs of one cell.
 struct MagRaw {
@@ -568,13 +584,14 @@ __device__ __forceinline__ void mag_load(const MagArgs& a, long c, MagRaw& m) {
   m.vk[2] = a.vk[2] ? a.vk[2][c] : 0.f;
 }
 
-// (avgB, E = avgB x avgv) of one cell (_magnetic_field_update.py:84,149-154; _vector_maths.py:5-10)
-__device__ __forceinline__ void mag_derive(const MagRaw& m, int euler, float* d) {
+// (avgB, E = avgB x avgv) of iterate (Bk, vk) (_magnetic_field_update.py:84,149-154; _vector_maths.py:5-10)
+__device__ __forceinline__ void mag_derive3(const float* Bk, const float* vk, const float* B0, const float* v0, int euler,
+                                            float* d) {
   float b[3], v[3];
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
-    b[i] = euler ? m.Bk[i] : (m.Bk[i] + m.B0[i]) * 0.5f;
-    v[i] = euler ? m.vk[i] : (m.vk[i] + m.v0[i]) * 0.5f;
+    b[i] = euler ? Bk[i] : (Bk[i] + B0[i]) * 0.5f;
+    v[i] = euler ? vk[i] : (vk[i] + v0[i]) * 0.5f;
   }
   d[0] = b[0];
   d[1] = b[1];
@@ -584,37 +601,97 @@ __device__ __forceinline__ void mag_derive(const MagRaw& m, int euler, float* d)
   d[5] = b[0] * v[1] - b[1] * v[0];
 }
 
+__device__ __forceinline__ void mag_derive(const MagRaw& m, int euler, float* d) {
+  mag_derive3(m.Bk, m.vk, m.B0, m.v0, euler, d);
+}
+
 // central derivative 0.5 (f+ - f-) / dx (_vector_maths.py:58-100)
 __device__ __forceinline__ float cdiff(float fp, float fm, float inv_dx) {
   return (0.5f * (fp - fm)) * inv_dx;
 }
 
-// One fixed-point iterate of the implicit-midpoint magnetic sub-step:
-// B' = B0 - dt curl(avgB x avgv), v' = v0 - dt (avgB x curl avgB) / rho,
-// with the residual max(|Bk - B'|, |vk - v'|) reduced into ctl->resp[k % 3]
-// (_magnetic_field_update.py:112-165).
-template <int DIM, int TA>
-__global__ __launch_bounds__(TA * TB) void k_mag(const MagArgs a) {
-  constexpr int NT = TA * TB, WW = TB + 2, PL = (TA + 2) * WW;
-  const int tid = threadIdx.x, ta = tid / TB, tb = tid % TB;
+// curl(E) and curl(avgB) at LDS offset o of a plane (row stride W) from the
+// derived fields S[0..5] = (avgB, E); x-neighbours dm, dp (3D)
+template <int DIM, int W>
+__device__ __forceinline__ void mag_curls(const float* const* S, int o, const float* dm, const float* dp, float idx,
+                                          float* cE, float* cB) {
+  if (DIM == 3) {
+    cE[0] = cdiff(S[5][o + W], S[5][o - W], idx) - cdiff(S[4][o + 1], S[4][o - 1], idx);
+    cE[1] = cdiff(S[3][o + 1], S[3][o - 1], idx) - cdiff(dp[5], dm[5], idx);
+    cE[2] = cdiff(dp[4], dm[4], idx) - cdiff(S[3][o + W], S[3][o - W], idx);
+    cB[0] = cdiff(S[2][o + W], S[2][o - W], idx) - cdiff(S[1][o + 1], S[1][o - 1], idx);
+    cB[1] = cdiff(S[0][o + 1], S[0][o - 1], idx) - cdiff(dp[2], dm[2], idx);
+    cB[2] = cdiff(dp[1], dm[1], idx) - cdiff(S[0][o + W], S[0][o - W], idx);
+  } else {
+    cE[0] = cdiff(S[5][o + 1], S[5][o - 1], idx);
+    cE[1] = -cdiff(S[5][o + W], S[5][o - W], idx);
+    cE[2] = cdiff(S[4][o + W], S[4][o - W], idx) - cdiff(S[3][o + 1], S[3][o - 1], idx);
+    cB[0] = cdiff(S[2][o + 1], S[2][o - 1], idx);
+    cB[1] = -cdiff(S[2][o + W], S[2][o - W], idx);
+    cB[2] = cdiff(S[1][o + W], S[1][o - W], idx) - cdiff(S[0][o + 1], S[0][o - 1], idx);
+  }
+}
+
+// One iterate: B' = B0 - dt curl E, v' = v0 - dt (avgB x curl avgB) / rho
+__device__ __forceinline__ void mag_apply(const float* avgB, const float* cE, const float* cB, const float* B0,
+                                          const float* v0, float rho, float dt, float* Bn, float* vn) {
+  const float ri = frcp(rho);
+  const float p2[3] = {(avgB[1] * cB[2] - avgB[2] * cB[1]) * ri, (avgB[2] * cB[0] - avgB[0] * cB[2]) * ri,
+                       (avgB[0] * cB[1] - avgB[1] * cB[0]) * ri};
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    Bn[i] = B0[i] - dt * cE[i];
+    vn[i] = v0[i] - dt * p2[i];
+  }
+}
+
+__device__ __forceinline__ float mag_resid(const float* Bk, const float* vk, const float* Bn, const float* vn) {
+  float nb = 0.f, nv = 0.f;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const float db = Bk[i] - Bn[i], dv = vk[i] - vn[i];
+    nb += db * db;
+    nv += dv * dv;
+  }
+  return fmaxf(fsqrt(nb), fsqrt(nv));
+}
+
+// Two fixed-point launches (L, L+1) of the implicit-midpoint magnetic
+// sub-step in one pass (_magnetic_field_update.py:82-165): iterate L+1 from
+// iterate L on the tile and a one-cell ring, then iterate L+2 on the tile
+// from those, so the pair reads the fixed-point inputs (B0, v0, rho) and the
+// input iterate once. The block covers the tile with a two-cell ring; it
+// marches along x with the second iterate one plane behind the first. The
+// ring cells of the first iterate are recomputed, bit-identically, by the
+// neighbouring blocks that own them. Both iterates are written; the residuals
+// go to res slots L and L+1. Launch L+1's iterate is speculative: whether it
+// is the reference's result is decided by pair_final.
+template <int DIM, int TA, int TBM>
+__global__ __launch_bounds__(((TA + 4) * (TBM + 4) + 255) / 256 * 256) void k_mag(const MagArgs a) {
+  constexpr int WX = TBM + 4, NC = (TA + 4) * WX;   // ring-2 tile
+  constexpr int W1 = TBM + 2, NC1 = (TA + 2) * W1;  // ring-1 tile
+  constexpr int NT = (NC + 255) / 256 * 256, NW = NT / 64;
+  const int tid = threadIdx.x;
   const int slot = a.slot0 + blockIdx.y;
   Ctl* const C = a.ctl + slot;
   if (!C->run) return;
-  if (a.k > 0) {
-    // while condition of _magnetic_field_update.py:127-141 on iterate k-1
+  const int L = a.L;
+  if (L > 0) {
     if (C->done) return;
-    const unsigned rb = wave_max_u(C->resp[(a.k - 1) % 3][tid & 63]);
-    const float r = __uint_as_float(rb);
-    if (!(r > a.tol) || a.k - 1 >= a.max_iter) {
+    const float r0 = __uint_as_float(wave_max_u(C->resp[(L - 2) % NRES][tid & 63]));
+    const float r1 = __uint_as_float(wave_max_u(C->resp[(L - 1) % NRES][tid & 63]));
+    if (pair_final(L - 2, r0, r1, a.tol, a.max_iter) >= 0) {
       if (blockIdx.x == 0 && tid == 0) C->done = 1;
       return;
     }
   }
-  __shared__ float sM[2][6][PL];
-  __shared__ unsigned sred[TA];
+  __shared__ float sK[2][6][NC];
+  __shared__ float sK1[2][6][NC1];
+  __shared__ unsigned sred[2][NW];
   if (blockIdx.x == 0 && tid < NSLOT) {
-    if (tid == 0) C->iters = a.k;
-    C->resp[(a.k + 1) % 3][tid] = 0u;
+    if (tid == 0) C->iters = L + 1;
+    C->resp[(L + 2) % NRES][tid] = 0u;
+    C->resp[(L + 3) % NRES][tid] = 0u;
   }
   const long so = slot * a.sstride;
   MagArgs m = a;  // slot-local pointers
@@ -622,10 +699,13 @@ __global__ __launch_bounds__(TA * TB) void k_mag(const MagArgs a) {
   for (int i = 0; i < 3; ++i) {
     m.B0[i] = a.B0[i] + so;
     m.Bk[i] = a.Bk[i] + so;
-    m.Bo[i] = a.Bo[i] + so;
-    m.vo[i] = a.vo[i] + so;
     m.v0[i] = a.v0[i] ? a.v0[i] + so : nullptr;
     m.vk[i] = a.vk[i] ? a.vk[i] + so : nullptr;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      m.Bo[j][i] = a.Bo[j][i] + so;
+      m.vo[j][i] = a.vo[j][i] + so;
+    }
   }
   m.rho = a.rho + so;
   const int bid = swizzle_block(blockIdx.x, gridDim.x);
@@ -633,124 +713,195 @@ __global__ __launch_bounds__(TA * TB) void k_mag(const MagArgs a) {
   const int rest = bid / a.ntb;
   const int ita = rest % a.nta;
   const int icx = rest / a.nta;
-  const int a0 = ita * TA, b0 = itb * TB;
+  const int a0 = ita * TA, b0 = itb * TBM;
   const long sa = a.nb, sx = (long)a.na * a.nb;
-  const int ga = wrapi(a0 + ta, a.na), gb = wrapi(b0 + tb, a.nb);
-  const bool active = (a0 + ta < a.na) && (b0 + tb < a.nb);
+  const bool has = tid < NC;
+  const int r = has ? tid / WX - 2 : 0, c = has ? tid % WX - 2 : 0;
+  const bool ring2 = r == -2 || r == TA + 1 || c == -2 || c == TBM + 1;
+  const bool inner = has && !ring2;                       // first iterate here
+  const bool tile = inner && r >= 0 && r < TA && c >= 0 && c < TBM;
+  const bool own = tile && (a0 + r < a.na) && (b0 + c < a.nb);
+  const long col = (long)wrapi(a0 + r, a.na) * sa + wrapi(b0 + c, a.nb);
+  const int o = (r + 2) * WX + (c + 2);
+  const int o1 = (r + 1) * W1 + (c + 1);
   const float dt = C->dt;
   const float idx = a.inv_dx;
-  const long col = (long)ga * sa + gb;
-  int x0 = 0, x1 = 1;
-  if (DIM == 3) {
-    x0 = icx * a.lx;
-    x1 = min(x0 + a.lx, a.nx);
-  }
-  MagRaw rc, rn;
-  float dm[6], dc[6], dp[6];
-  if (DIM == 3) {
-    MagRaw t;
-    mag_load(m, (long)wrapi(x0 - 1, a.nx) * sx + col, t);
-    mag_derive(t, a.euler, dm);
-    mag_load(m, (long)x0 * sx + col, rc);
-    mag_derive(rc, a.euler, dc);
-    mag_load(m, (long)wrapi(x0 + 1, a.nx) * sx + col, rn);
-    mag_derive(rn, a.euler, dp);
-  } else {
-    mag_load(m, col, rc);
-    mag_derive(rc, a.euler, dc);
-  }
-  // this thread's halo cell of the plane ring (rows -1, TA; columns -1, TB)
-  constexpr int NH = 2 * TB + 2 * TA;
-  int hr = 0, hc = 0;
-  long hoff = 0;
-  if (tid < NH) {
-    if (tid < 2 * TB) {
-      hr = tid < TB ? -1 : TA;
-      hc = tid % TB;
-    } else {
-      const int j = tid - 2 * TB;
-      hr = j >> 1;
-      hc = (j & 1) ? TB : -1;
-    }
-    hoff = (long)wrapi(a0 + hr, a.na) * sa + wrapi(b0 + hc, a.nb);
-  }
-  float res = 0.f;
-  int buf = 0;
-  // halo loads run one plane ahead of their use
-  MagRaw th;
-  if (tid < NH) mag_load(m, (DIM == 3 ? (long)x0 * sx : 0) + hoff, th);
-  for (int x = x0; x < x1; ++x) {
-    const long plane = DIM == 3 ? (long)x * sx : 0;
-    // plane fill: own cell and the halo ring
+  float res0 = 0.f, res1 = 0.f;
+  const float* SK[6];
+  const float* SK1[6];
+
+  if (DIM == 2) {
+    MagRaw rc;
+    float d[6], rho = 1.f;
+    if (has) {
+      mag_load(m, col, rc);
+      mag_derive(rc, a.euler, d);
 #pragma unroll
-    for (int i = 0; i < 6; ++i) sM[buf][i][(ta + 1) * WW + tb + 1] = dc[i];
-    if (tid < NH) {
-      float d[6];
-      mag_derive(th, a.euler, d);
-#pragma unroll
-      for (int i = 0; i < 6; ++i) sM[buf][i][(hr + 1) * WW + hc + 1] = d[i];
-      if (DIM == 3 && x + 1 < x1) mag_load(m, (long)(x + 1) * sx + hoff, th);
+      for (int i = 0; i < 6; ++i) sK[0][i][o] = d[i];
     }
-    MagRaw rnn;
-    float dpp[6];
-    if (DIM == 3) {
-      // prefetch the next column cell
-      mag_load(m, (long)wrapi(x + 2, a.nx) * sx + col, rnn);
+    if (inner) rho = m.rho[col];
+    __syncthreads();
+    float B1[3], v1[3];
+    if (inner) {
+      float cE[3], cB[3], e1[6];
+#pragma unroll
+      for (int i = 0; i < 6; ++i) SK[i] = sK[0][i];
+      mag_curls<2, WX>(SK, o, nullptr, nullptr, idx, cE, cB);
+      mag_apply(d, cE, cB, rc.B0, rc.v0, rho, dt, B1, v1);
+      if (own) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          m.Bo[0][i][col] = B1[i];
+          m.vo[0][i][col] = v1[i];
+        }
+        const float rr = mag_resid(rc.Bk, rc.vk, B1, v1);
+        res0 = (rr != rr || rr > res0) ? rr : res0;
+      }
+      mag_derive3(B1, v1, rc.B0, rc.v0, a.euler, e1);
+#pragma unroll
+      for (int i = 0; i < 6; ++i) sK1[0][i][o1] = e1[i];
     }
     __syncthreads();
-    if (active) {
-      const long cell = plane + col;
-      const int o = (ta + 1) * WW + tb + 1;
-      float (*S)[PL] = sM[buf];
-      // neighbour values: A +/- 1 rows, B +/- 1 columns
-      float cE[3], cB[3];
-      if (DIM == 3) {
-        cE[0] = cdiff(S[5][o + WW], S[5][o - WW], idx) - cdiff(S[4][o + 1], S[4][o - 1], idx);
-        cE[1] = cdiff(S[3][o + 1], S[3][o - 1], idx) - cdiff(dp[5], dm[5], idx);
-        cE[2] = cdiff(dp[4], dm[4], idx) - cdiff(S[3][o + WW], S[3][o - WW], idx);
-        cB[0] = cdiff(S[2][o + WW], S[2][o - WW], idx) - cdiff(S[1][o + 1], S[1][o - 1], idx);
-        cB[1] = cdiff(S[0][o + 1], S[0][o - 1], idx) - cdiff(dp[2], dm[2], idx);
-        cB[2] = cdiff(dp[1], dm[1], idx) - cdiff(S[0][o + WW], S[0][o - WW], idx);
-      } else {
-        cE[0] = cdiff(S[5][o + 1], S[5][o - 1], idx);
-        cE[1] = -cdiff(S[5][o + WW], S[5][o - WW], idx);
-        cE[2] = cdiff(S[4][o + WW], S[4][o - WW], idx) - cdiff(S[3][o + 1], S[3][o - 1], idx);
-        cB[0] = cdiff(S[2][o + 1], S[2][o - 1], idx);
-        cB[1] = -cdiff(S[2][o + WW], S[2][o - WW], idx);
-        cB[2] = cdiff(S[1][o + WW], S[1][o - WW], idx) - cdiff(S[0][o + 1], S[0][o - 1], idx);
-      }
-      const float ri = frcp(m.rho[cell]);
-      const float p2[3] = {(dc[1] * cB[2] - dc[2] * cB[1]) * ri, (dc[2] * cB[0] - dc[0] * cB[2]) * ri,
-                           (dc[0] * cB[1] - dc[1] * cB[0]) * ri};
-      float nb = 0.f, nv = 0.f;
-#pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        const float bn = rc.B0[i] - dt * cE[i];
-        const float vn = rc.v0[i] - dt * p2[i];
-        m.Bo[i][cell] = bn;
-        m.vo[i][cell] = vn;
-        const float db = rc.Bk[i] - bn, dv = rc.vk[i] - vn;
-        nb += db * db;
-        nv += dv * dv;
-      }
-      const float r = fmaxf(fsqrt(nb), fsqrt(nv));
-      res = (r != r || r > res) ? r : res;
-    }
-    if (DIM == 3) {
-      mag_derive(rnn, a.euler, dpp);
+    if (own) {
+      float cE[3], cB[3], e1[6], B2[3], v2[3];
 #pragma unroll
       for (int i = 0; i < 6; ++i) {
-        dm[i] = dc[i];
-        dc[i] = dp[i];
-        dp[i] = dpp[i];
+        SK1[i] = sK1[0][i];
+        e1[i] = sK1[0][i][o1];
       }
-      rc = rn;
-      rn = rnn;
+      mag_curls<2, W1>(SK1, o1, nullptr, nullptr, idx, cE, cB);
+      mag_apply(e1, cE, cB, rc.B0, rc.v0, rho, dt, B2, v2);
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        m.Bo[1][i][col] = B2[i];
+        m.vo[1][i][col] = v2[i];
+      }
+      const float rr = mag_resid(B1, v1, B2, v2);
+      res1 = (rr != rr || rr > res1) ? rr : res1;
     }
-    buf ^= 1;
+  } else {
+    const int x0 = icx * a.lx, x1 = min(x0 + a.lx, a.nx);
+    float dm[6], dc[6], dp[6];
+    MagRaw rc, rn;
+    float rho_c = 1.f, rho_n = 1.f;
+    if (has) {
+      MagRaw t;
+      mag_load(m, (long)wrapi(x0 - 2, a.nx) * sx + col, t);
+      mag_derive(t, a.euler, dm);
+      mag_load(m, (long)wrapi(x0 - 1, a.nx) * sx + col, rc);
+      mag_derive(rc, a.euler, dc);
+      mag_load(m, (long)x0 * sx + col, rn);
+      mag_derive(rn, a.euler, dp);
+    }
+    if (inner) {
+      rho_c = m.rho[(long)wrapi(x0 - 1, a.nx) * sx + col];
+      rho_n = m.rho[(long)x0 * sx + col];
+    }
+    // second-iterate carries of the own column (tile threads): derived first
+    // iterate at x-2, x-1; first iterate and fixed-point inputs at x-1
+    float e1m[6], e1c[6], Bp[3], vp[3], B0p[3], v0p[3], rhop = 1.f;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) e1m[i] = e1c[i] = 0.f;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) Bp[i] = vp[i] = B0p[i] = v0p[i] = 0.f;
+    for (int x = x0 - 1; x <= x1; ++x) {
+      const int bx = x & 1;
+      // prefetch x + 2
+      MagRaw rnn;
+      float rho_nn = 1.f;
+      const long pnn = (long)wrapi(x + 2, a.nx) * sx + col;
+      if (has) mag_load(m, pnn, rnn);
+      if (inner) rho_nn = m.rho[pnn];
+      if (has) {
+#pragma unroll
+        for (int i = 0; i < 6; ++i) sK[bx][i][o] = dc[i];
+      }
+      __syncthreads();
+      float B1[3], v1[3], e1[6];
+      if (inner) {
+        float cE[3], cB[3];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) SK[i] = sK[bx][i];
+        mag_curls<3, WX>(SK, o, dm, dp, idx, cE, cB);
+        mag_apply(dc, cE, cB, rc.B0, rc.v0, rho_c, dt, B1, v1);
+        if (own && x >= x0 && x < x1) {
+          const long cell = (long)x * sx + col;
+#pragma unroll
+          for (int i = 0; i < 3; ++i) {
+            m.Bo[0][i][cell] = B1[i];
+            m.vo[0][i][cell] = v1[i];
+          }
+          const float rr = mag_resid(rc.Bk, rc.vk, B1, v1);
+          res0 = (rr != rr || rr > res0) ? rr : res0;
+        }
+        mag_derive3(B1, v1, rc.B0, rc.v0, a.euler, e1);
+#pragma unroll
+        for (int i = 0; i < 6; ++i) sK1[bx][i][o1] = e1[i];
+      }
+      __syncthreads();
+      if (own && x - 1 >= x0 && x - 1 < x1) {
+        float cE[3], cB[3], B2[3], v2[3];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) SK1[i] = sK1[bx ^ 1][i];
+        mag_curls<3, W1>(SK1, o1, e1m, e1, idx, cE, cB);
+        mag_apply(e1c, cE, cB, B0p, v0p, rhop, dt, B2, v2);
+        const long cell = (long)(x - 1) * sx + col;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          m.Bo[1][i][cell] = B2[i];
+          m.vo[1][i][cell] = v2[i];
+        }
+        const float rr = mag_resid(Bp, vp, B2, v2);
+        res1 = (rr != rr || rr > res1) ? rr : res1;
+      }
+      if (tile) {
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+          e1m[i] = e1c[i];
+          e1c[i] = e1[i];
+        }
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          Bp[i] = B1[i];
+          vp[i] = v1[i];
+          B0p[i] = rc.B0[i];
+          v0p[i] = rc.v0[i];
+        }
+        rhop = rho_c;
+      }
+      if (has) {
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+          dm[i] = dc[i];
+          dc[i] = dp[i];
+        }
+        mag_derive(rnn, a.euler, dp);
+        rc = rn;
+        rn = rnn;
+      }
+      rho_c = rho_n;
+      rho_n = rho_nn;
+    }
   }
-  const unsigned mr = block_max_u<TA>(canon_bits(res), sred);
-  if (tid == 0 && mr) atomicMax(&C->resp[a.k % 3][blockIdx.x & (NSLOT - 1)], mr);
+  const unsigned m0 = block_max_u<NW>(canon_bits(res0), sred[0]);
+  const unsigned m1 = block_max_u<NW>(canon_bits(res1), sred[1]);
+  if (tid == 0) {
+    if (m0) atomicMax(&C->resp[L % NRES][blockIdx.x & (NSLOT - 1)], m0);
+    if (m1) atomicMax(&C->resp[(L + 1) % NRES][blockIdx.x & (NSLOT - 1)], m1);
+  }
+}
+
+// Final launch index of the fixed point of each running slot (one thread per
+// slot) from its last executed pair, recorded in ctl->iters: the stage
+// kernels after the sub-step read the iterate of that launch.
+__global__ void k_mag_final(Ctl* ctl, float tol, int max_iter) {
+  Ctl* const c = ctl + blockIdx.x;
+  if (!c->run) return;
+  const int L = c->iters - 1;
+  const float r0 = __uint_as_float(slots_max(c->resp[L % NRES]));
+  const float r1 = __uint_as_float(slots_max(c->resp[(L + 1) % NRES]));
+  const int f = pair_final(L, r0, r1, tol, max_iter);
+  c->iters = f >= 0 ? f : L + 1;
 }
 
 // Unsplit CFL maxima max(|u_a| + c) per axis (_timestep_estimator.py:118-126)
@@ -820,7 +971,7 @@ __device__ __forceinline__ float cfl_dt_from(const Ctl* c, float C, float dx, in
 }
 
 __device__ __forceinline__ void fixed_point_reset(Ctl* c) {
-  for (int i = 0; i < NSLOT; ++i) c->resp[0][i] = 0u;
+  for (int i = 0; i < NSLOT; ++i) c->resp[0][i] = c->resp[1][i] = 0u;
   c->iters = 0;
   c->done = 0;
 }
@@ -886,7 +1037,9 @@ __global__ void k_slot_reset(Ctl* ctl, int slot0, float t_end, int live) {
   c->steps = 0;
   for (int i = 0; i < 4; ++i) c->cfl[i] = 0u;
   for (int k = 0; k < 3; ++k)
-    for (int i = 0; i < NSLOT; ++i) c->cflp[k][i] = c->resp[k][i] = 0u;
+    for (int i = 0; i < NSLOT; ++i) c->cflp[k][i] = 0u;
+  for (int k = 0; k < NRES; ++k)
+    for (int i = 0; i < NSLOT; ++i) c->resp[k][i] = 0u;
   c->iters = 0;
   c->done = 0;
   c->live = live;
@@ -934,10 +1087,17 @@ amx_status fail(amx_status s, const char* fmt, const char* what = "") {
 #define AMX_STAGE_TA 7
 #endif
 constexpr int TA_DEF = AMX_STAGE_TA;
+// Tile of the pair kernel k_mag: TA_MAG rows x TB_MAG columns; the block
+// covers the tile and a two-cell ring, (TA + 4) x (TB + 4) cells rounded up
+// to whole 256-thread groups
 #ifndef AMX_MAG_TA
-#define AMX_MAG_TA 8
+#define AMX_MAG_TA 7
+#endif
+#ifndef AMX_MAG_TB
+#define AMX_MAG_TB 64
 #endif
 constexpr int TA_MAG = AMX_MAG_TA;
+constexpr int TB_MAG = AMX_MAG_TB;
 
 }  // namespace
 
@@ -952,7 +1112,7 @@ struct amx_fv_ctx {
   long ncell;
   float dx;
   int na, nb, nxm;  // kernel axes: 3D (ny, nz, nx); 2D (nx, ny, 1)
-  int ntb, nta, nta_mag, ncx, lx, ncx_mag, lx_mag;
+  int ntb, nta, ntb_mag, nta_mag, ncx, lx, ncx_mag, lx_mag;
   int cap;
   long sstride;     // floats per slot (0 for a single simulation)
   float* mem;       // pool: cap x [state NS | G1 NV | P0 6 | P1 6]; single: [G1 | P0 | P1]
@@ -1050,7 +1210,8 @@ amx_status validate(const amx_fv_config* c) {
   return AMX_OK;
 }
 
-long ws_fields(const amx_fv_config* cfg) { return (cfg->dim + 2) + (cfg->mhd ? 12 : 0); }
+// G1 (gas rows) | P0 | P1 | P2 (B, v iterates of the fixed point)
+long ws_fields(const amx_fv_config* cfg) { return (cfg->dim + 2) + (cfg->mhd ? 18 : 0); }
 long state_fields(const amx_fv_config* cfg) { return (cfg->dim + 2) + (cfg->mhd ? 3 : 0); }
 long cells(const amx_fv_config* cfg) { return (long)cfg->nx * cfg->ny * (cfg->dim == 3 ? cfg->nz : 1); }
 
@@ -1104,6 +1265,7 @@ amx_status make_ctx(const amx_fv_config* cfg, int cap, int own_state, amx_fv_ctx
   c->ntb = (c->nb + TB - 1) / TB;
   c->nta = (c->na + TA_DEF - 1) / TA_DEF;
   c->nta_mag = (c->na + TA_MAG - 1) / TA_MAG;
+  c->ntb_mag = (c->nb + TB_MAG - 1) / TB_MAG;
   if (cfg->dim == 3) {
     // planes per block along x: aim for >= ~`blocks` blocks with chunks of
     // >= 8 planes, rounded down to a power of two (the stencil kernels reload
@@ -1119,7 +1281,7 @@ amx_status make_ctx(const amx_fv_config* cfg, int cap, int own_state, amx_fv_ctx
       return lx > cfg->nx ? cfg->nx : lx;
     };
     c->lx = pick((long)c->ntb * c->nta, "AMX_LX", 2048);
-    c->lx_mag = pick((long)c->ntb * c->nta_mag, "AMX_LX_MAG", 1024);
+    c->lx_mag = pick((long)c->ntb_mag * c->nta_mag, "AMX_LX_MAG", 1024);
   } else {
     c->lx = c->lx_mag = 1;
   }
@@ -1166,7 +1328,7 @@ struct Bufs {
   float* gas[5];
   float* B[3];
   float* G1;
-  float* P[2][6];
+  float* P[3][6];  // launch k of the fixed point writes P[k % 3]
 };
 
 Bufs bufs(const amx_fv_ctx* c, float* state) {
@@ -1179,8 +1341,9 @@ Bufs bufs(const amx_fv_ctx* c, float* state) {
     for (int k = 0; k < 3; ++k) b.B[k] = state + (long)(c->NV + k) * n;
   float* ws = c->own_state ? c->mem + (long)c->NS * n : c->mem;
   b.G1 = ws;
-  for (int j = 0; j < 2; ++j)
-    for (int k = 0; k < 6; ++k) b.P[j][k] = ws + (long)(c->NV + 6 * j + k) * n;
+  if (c->cfg.mhd)
+    for (int j = 0; j < 3; ++j)
+      for (int k = 0; k < 6; ++k) b.P[j][k] = ws + (long)(c->NV + 6 * j + k) * n;
   return b;
 }
 
@@ -1230,7 +1393,7 @@ MagArgs mag_base(const amx_fv_ctx* c) {
   m.nx = c->nxm;
   m.na = c->na;
   m.nb = c->nb;
-  m.ntb = c->ntb;
+  m.ntb = c->ntb_mag;
   m.nta = c->nta_mag;
   m.ncx = c->ncx_mag;
   m.lx = c->lx_mag;
@@ -1239,11 +1402,13 @@ MagArgs mag_base(const amx_fv_ctx* c) {
 }
 
 hipError_t launch_mag(const amx_fv_ctx* c, const MagArgs& m, int nspan, hipStream_t st) {
-  const dim3 grid(c->ntb * c->nta_mag * c->ncx_mag, nspan), block(TA_MAG * TB);
+  constexpr int NT = ((TA_MAG + 4) * (TB_MAG + 4) + 255) / 256 * 256;
+  static_assert(NT <= 1024, "pair-kernel block too large");
+  const dim3 grid(c->ntb_mag * c->nta_mag * c->ncx_mag, nspan), block(NT);
   if (c->cfg.dim == 3)
-    hipLaunchKernelGGL((k_mag<3, TA_MAG>), grid, block, 0, st, m);
+    hipLaunchKernelGGL((k_mag<3, TA_MAG, TB_MAG>), grid, block, 0, st, m);
   else
-    hipLaunchKernelGGL((k_mag<2, TA_MAG>), grid, block, 0, st, m);
+    hipLaunchKernelGGL((k_mag<2, TA_MAG, TB_MAG>), grid, block, 0, st, m);
   return hipGetLastError();
 }
 
@@ -1266,6 +1431,39 @@ hipError_t launch_cfl(const amx_fv_ctx* c, const float* state, int slot0, int ns
   else
     hipLaunchKernelGGL(k_cfl<2>, dim3(blocks, nspan), dim3(256), 0, st, a);
   return hipGetLastError();
+}
+
+// Launches the fixed-point pair (L, L+1): input = launch L-1's output (the
+// state's B and gas velocity for L = 0), outputs to P[L % 3], P[(L+1) % 3].
+hipError_t launch_mag_pair(amx_fv_ctx* c, const Bufs& b, int L, int nspan, hipStream_t st) {
+  const int dim = c->cfg.dim;
+  MagArgs m = mag_base(c);
+  for (int k = 0; k < 3; ++k) m.B0[k] = b.B[k];
+  m.v0[0] = b.gas[1];
+  m.v0[1] = b.gas[2];
+  m.v0[2] = dim == 3 ? b.gas[3] : nullptr;
+  m.rho = b.gas[0];
+  if (L == 0) {
+    for (int k = 0; k < 3; ++k) {
+      m.Bk[k] = m.B0[k];
+      m.vk[k] = m.v0[k];
+    }
+  } else {
+    for (int k = 0; k < 3; ++k) {
+      m.Bk[k] = b.P[(L - 1) % 3][k];
+      m.vk[k] = b.P[(L - 1) % 3][3 + k];
+    }
+  }
+  for (int j = 0; j < 2; ++j)
+    for (int k = 0; k < 3; ++k) {
+      m.Bo[j][k] = b.P[(L + j) % 3][k];
+      m.vo[j][k] = b.P[(L + j) % 3][3 + k];
+    }
+  m.L = L;
+  const int e = time_begin(c, st);
+  const hipError_t err = launch_mag(c, m, nspan, st);
+  time_end(c, st, e, CLS_MAG, L);
+  return err;
 }
 
 // ---- one step of slots [0, nspan), dt already in ctl[s].dt, ctl[s].run set.
@@ -1314,96 +1512,63 @@ amx_status step_phase1(amx_fv_ctx* c, const Bufs& b, int nspan, hipStream_t st, 
   e = time_begin(c, st);
   HIPCHK((launch_stage<SRC_PLAIN, true, SRC_PLAIN>(c, a2, nspan, st)));
   time_end(c, st, e, CLS_STAGE, 0);
-  // ---- S_B(dt): fixed point on (B, v) with rho from the gas
-  MagArgs m = mag_base(c);
-  for (int k = 0; k < 3; ++k) {
-    m.B0[k] = b.B[k];
-    m.Bk[k] = b.B[k];
-  }
-  m.v0[0] = b.gas[1];
-  m.v0[1] = b.gas[2];
-  m.v0[2] = dim == 3 ? b.gas[3] : nullptr;
-  m.vk[0] = m.v0[0];
-  m.vk[1] = m.v0[1];
-  m.vk[2] = m.v0[2];
-  m.rho = b.gas[0];
-  for (int k = 0; k < 3; ++k) {
-    m.Bo[k] = b.P[0][k];
-    m.vo[k] = b.P[0][3 + k];
-  }
-  m.k = 0;
-  e = time_begin(c, st);
-  HIPCHK(launch_mag(c, m, nspan, st));
-  time_end(c, st, e, CLS_MAG, 0);
-  int batch = c->last_k + 1;
-  if (batch < 1) batch = 1;
+  // ---- S_B(dt): fixed point on (B, v) with rho from the gas, in pairs of
+  // launches; the speculative batch covers the last step's launch count
   const int maxit = c->cfg.max_fixed_point_iterations;
-  *klast = batch < maxit ? batch : maxit;
-  for (int k = 1; k <= *klast; ++k) {
-    MagArgs mk = m;
-    mk.k = k;
-    for (int i = 0; i < 3; ++i) {
-      mk.Bk[i] = b.P[(k - 1) & 1][i];
-      mk.vk[i] = b.P[(k - 1) & 1][3 + i];
-      mk.Bo[i] = b.P[k & 1][i];
-      mk.vo[i] = b.P[k & 1][3 + i];
-    }
-    const int ek = time_begin(c, st);
-    HIPCHK(launch_mag(c, mk, nspan, st));
-    time_end(c, st, ek, CLS_MAG, k);
+  int upto = c->last_k < 1 ? 1 : c->last_k;
+  if (upto > maxit) upto = maxit;
+  *klast = -1;
+  for (int L = 0; L <= upto; L += 2) {
+    HIPCHK(launch_mag_pair(c, b, L, nspan, st));
+    *klast = L + 1;
   }
   return AMX_OK;
 }
 
-// Fixed-point state of the slots after a readback of ctl[0, nspan) into hctl:
-// returns 1 when every running slot has converged (or hit the cap) by klast.
-int fixed_point_settled(const amx_fv_ctx* c, int nspan, int klast, int* kmax) {
-  const int maxit = c->cfg.max_fixed_point_iterations;
+// Final launch index of a running slot's fixed point from a readback of its
+// control block, or -1 if the reference's loop would still run.
+int slot_final(const amx_fv_ctx* c, const Ctl& h) {
+  const int L = h.iters - 1;  // last executed pair
+  unsigned r0 = 0, r1 = 0;
+  for (int i = 0; i < NSLOT; ++i) {
+    r0 = h.resp[L % NRES][i] > r0 ? h.resp[L % NRES][i] : r0;
+    r1 = h.resp[(L + 1) % NRES][i] > r1 ? h.resp[(L + 1) % NRES][i] : r1;
+  }
+  float f0, f1;
+  memcpy(&f0, &r0, sizeof(float));
+  memcpy(&f1, &r1, sizeof(float));
+  return pair_final(L, f0, f1, c->cfg.fixed_point_tol, c->cfg.max_fixed_point_iterations);
+}
+
+// After a readback of ctl[0, nspan) into hctl: 1 when every running slot has
+// settled; *kmax = largest final launch index.
+int fixed_point_settled(const amx_fv_ctx* c, int nspan, int* kmax) {
   int settled = 1;
   *kmax = 0;
   for (int s = 0; s < nspan; ++s) {
     const Ctl& h = c->hctl[s];
     if (!h.run) continue;
-    const int K = h.iters;
-    if (K > *kmax) *kmax = K;
-    if (K < klast || K >= maxit) continue;  // a later launch exited: converged; or capped
-    unsigned rb = 0;
-    for (int i = 0; i < NSLOT; ++i) rb = h.resp[K % 3][i] > rb ? h.resp[K % 3][i] : rb;
-    float r;
-    memcpy(&r, &rb, sizeof(float));
-    if (r > c->cfg.fixed_point_tol) settled = 0;
+    int f = slot_final(c, h);
+    if (f < 0) {
+      settled = 0;
+      f = h.iters;
+    }
+    if (f > *kmax) *kmax = f;
   }
   return settled;
 }
 
-// Launches fixed-point iterations klast+1, klast+2, ... (two per round) with a
-// readback after each round until every running slot has settled.
+// Launches further pairs after klast (two pairs per round) with a readback
+// after each round until every running slot has settled.
 amx_status fixed_point_more(amx_fv_ctx* c, const Bufs& b, int nspan, hipStream_t st, int* klast) {
   const int maxit = c->cfg.max_fixed_point_iterations;
-  MagArgs m = mag_base(c);
-  const int dim = c->cfg.dim;
-  for (int k = 0; k < 3; ++k) m.B0[k] = b.B[k];
-  m.v0[0] = b.gas[1];
-  m.v0[1] = b.gas[2];
-  m.v0[2] = dim == 3 ? b.gas[3] : nullptr;
-  m.rho = b.gas[0];
   int kmax = 0;
-  while (!fixed_point_settled(c, nspan, *klast, &kmax) && *klast < maxit) {
-    const int k1 = *klast + 1, k2 = *klast + 2 < maxit ? *klast + 2 : maxit;
-    for (int k = k1; k <= k2; ++k) {
-      MagArgs mk = m;
-      mk.k = k;
-      for (int i = 0; i < 3; ++i) {
-        mk.Bk[i] = b.P[(k - 1) & 1][i];
-        mk.vk[i] = b.P[(k - 1) & 1][3 + i];
-        mk.Bo[i] = b.P[k & 1][i];
-        mk.vo[i] = b.P[k & 1][3 + i];
-      }
-      const int ek = time_begin(c, st);
-      HIPCHK(launch_mag(c, mk, nspan, st));
-      time_end(c, st, ek, CLS_MAG, k);
+  while (!fixed_point_settled(c, nspan, &kmax) && *klast < maxit) {
+    for (int r = 0; r < 2 && *klast < maxit; ++r) {
+      const int L = *klast + 1;
+      HIPCHK(launch_mag_pair(c, b, L, nspan, st));
+      *klast = L + 1;
     }
-    *klast = k2;
     HIPCHK(hipMemcpyAsync(c->hctl, c->ctl, sizeof(Ctl) * nspan, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
   }
@@ -1415,12 +1580,15 @@ amx_status fixed_point_more(amx_fv_ctx* c, const Bufs& b, int nspan, hipStream_t
 amx_status step_phase3(amx_fv_ctx* c, const Bufs& b, int nspan, hipStream_t st, int want_cfl) {
   if (!c->cfg.mhd) return AMX_OK;
   const int NV = c->NV;
+  hipLaunchKernelGGL(k_mag_final, dim3(nspan), dim3(1), 0, st, c->ctl, c->cfg.fixed_point_tol,
+                     c->cfg.max_fixed_point_iterations);
+  HIPCHK(hipGetLastError());
   StageArgs b1 = stage_base(c, 0.5f);
   for (int v = 0; v < NV; ++v) {
     b1.src[v] = b.gas[v];
     b1.out[v] = b.G1 + (long)v * c->ncell;
   }
-  for (int j = 0; j < 2; ++j)
+  for (int j = 0; j < 3; ++j)
     for (int k = 0; k < 3; ++k) {
       b1.selv[j][k] = b.P[j][3 + k];
       b1.selb[j][k] = b.P[j][k];
@@ -1434,7 +1602,7 @@ amx_status step_phase3(amx_fv_ctx* c, const Bufs& b, int nspan, hipStream_t st, 
     b2.w0[v] = b.gas[v];
     b2.out[v] = b.gas[v];
   }
-  for (int j = 0; j < 2; ++j)
+  for (int j = 0; j < 3; ++j)
     for (int k = 0; k < 3; ++k) {
       b2.selv[j][k] = b.P[j][3 + k];
       b2.selb[j][k] = b.P[j][k];
@@ -1461,7 +1629,7 @@ amx_status do_step(amx_fv_ctx* c, float* state, hipStream_t st, int want_cfl, in
     HIPCHK(hipStreamSynchronize(st));
     s = fixed_point_more(c, b, 1, st, &klast);
     if (s != AMX_OK) return s;
-    K = c->hctl[0].iters;
+    fixed_point_settled(c, 1, &K);
     time_resolve(c, K);
     c->last_k = K;
   }
@@ -1853,11 +2021,11 @@ extern "C" amx_status amx_fv_pool_step_launch(amx_fv_pool* p, void* stream) {
   const Bufs b = bufs(c, nullptr);
   int klast = 0;
   if (c->cfg.mhd) {
-    // the speculative fixed-point batch of step_phase1
+    // the speculative fixed-point batch of step_phase1: pairs L = 0, 2, ... <= upto
     const int maxit = c->cfg.max_fixed_point_iterations;
-    klast = c->last_k + 1 < 1 ? 1 : c->last_k + 1;
-    if (klast > maxit) klast = maxit;
-    c->last_k = klast - 1;
+    int upto = c->last_k < 1 ? 1 : c->last_k;
+    if (upto > maxit) upto = maxit;
+    klast = (upto / 2) * 2 + 1;
   }
   const auto key = std::make_pair(span, klast);
   auto it = c->g_launch.find(key);
@@ -1891,10 +2059,10 @@ extern "C" amx_status amx_fv_pool_step_finish(amx_fv_pool* p, void* stream) {
   if (c->cfg.mhd) {
     amx_status s = fixed_point_more(c, b, span, st, &klast);
     if (s != AMX_OK) return s;
-    fixed_point_settled(c, span, klast, &kmax);
+    fixed_point_settled(c, span, &kmax);
     c->last_k = kmax;
     for (int i = 0; i < span; ++i)
-      if (c->hctl[i].run) c->iters_total[i] += c->hctl[i].iters;
+      if (c->hctl[i].run) c->iters_total[i] += slot_final(c, c->hctl[i]);
   }
   auto it = c->g_finish.find(span);
   hipGraphExec_t x = it == c->g_finish.end() ? nullptr : it->second;
