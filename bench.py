@@ -25,13 +25,15 @@ if HERE not in sys.path:
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E datasheet peak
 # Algorithmic bytes per cell of one launch (float32 fields read + written once):
-#   k_mag   : reads B_k, v_k, B0, v0, rho (13 fields), writes B_k+1, v_k+1 (6)  -> 76 B
+#   k_mag   : one launch makes two fixed-point iterates; the first pair reads
+#             B0, v0, rho (7 fields: the input iterate is B0, v0) and writes both
+#             iterates (12) -> 19 fields; later pairs also read the input iterate
+#             (6) -> 25 fields
 #   k_stage : mean over the four launches of a step
 #             A1 (R5 W5) + A2 (R10 W5) + A1' (R8 W5) + A2' (R16 W8) = 62 fields -> 62 B
-MAG_BYTES_PER_CELL = 76.0
+MAG_FIELDS_FIRST, MAG_FIELDS_LATER = 19.0, 25.0
 STAGE_BYTES_PER_CELL = 62.0
-# Whole step: 4 stage passes (62 fields) + magnetic init (13) + 19 per iteration
-STEP_FIELDS_BASE = 62 + 13
+STEP_FIELDS_STAGES = 62
 
 
 def cpu_baseline(n: int, steps: int):
@@ -394,7 +396,12 @@ def main():
     steps = st.steps
     value = steps * cells / el / 1e6
     kbar = st.fixed_point_iterations / max(steps, 1)
-    step_bytes = (STEP_FIELDS_BASE + 19.0 * kbar) * 4.0
+    # executed pair launches per step (from the event records; without them,
+    # the mean of floor(K / 2) + 1 is approximated by K / 2 + 1)
+    pairs = tm.mag_launches / max(steps, 1) if tm.mag_launches else kbar / 2.0 + 1.0
+    mag_fields = MAG_FIELDS_FIRST + MAG_FIELDS_LATER * max(pairs - 1.0, 0.0)
+    step_bytes = (STEP_FIELDS_STAGES + mag_fields) * 4.0
+    MAG_BYTES_PER_CELL = mag_fields * 4.0 / max(pairs, 1.0)
     mag_avg_ms = tm.mag_ms / max(tm.mag_launches, 1)
     stage_avg_ms = tm.stage_ms / max(tm.stage_launches, 1)
     if tm.mag_ms >= tm.stage_ms:
@@ -441,6 +448,7 @@ def main():
         },
         "step_model": {
             "fixed_point_iterations_per_step": round(kbar, 3),
+            "fixed_point_pair_launches_per_step": round(pairs, 3),
             "bytes_per_zone_update": round(step_bytes, 1),
             "effective_GBs": round(step_bytes * cells * steps / el / 1e9, 1),
             "frac_of_peak": round(step_bytes * cells * steps / el / 1e9 / HBM_PEAK_GBS, 4),

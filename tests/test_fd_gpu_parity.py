@@ -200,10 +200,11 @@ def _face_div(b):
 
 
 def test_fd_invariants_at_scale(gpu):
-    """128^3 turbulence, 3 steps: CT keeps the 6th-order face divergence at
-    float32 round-off; the flux form conserves mass and the face sums."""
-    print("[gpu] fd invariants 128^3", flush=True)
-    state = W.turbulence_3d(128, seed=9)
+    """The FD turbulence configuration at its full size (256^3, seed 0), 3
+    steps: CT keeps the 6th-order face divergence at float32 round-off; the
+    flux form conserves mass and the face sums."""
+    print("[gpu] fd invariants 256^3", flush=True)
+    state = W.turbulence_3d(256, seed=0)
     params = SimulationParams(C_cfl=0.4, t_end=1e9, gamma=5.0 / 3.0)
     out, st = gpu_run(state, params, max_steps=3)
     assert st.steps == 3
