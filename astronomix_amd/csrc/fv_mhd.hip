@@ -223,6 +223,13 @@ enum { SRC_PLAIN = 0, SRC_MAGFIX = 1 };
 // state entering the magnetic sub-step plus the converged velocity vn (slots
 // 5..7) and applies the sub-step's write-back: v <- vn and the energy fix with
 // gamma = 5/3 (_magnetic_field_update.py:208-240).
+// Element c of a field through a 32-bit byte offset (c < 2^30): the load
+// takes the scalar base plus one 32-bit vector offset, no 64-bit address math.
+__device__ __forceinline__ float ld32(const float* base, unsigned c) {
+  return *(const float*)((const char*)base + (c << 2));
+}
+__device__ __forceinline__ void st32(float* base, unsigned c, float v) { *(float*)((char*)base + (c << 2)) = v; }
+
 template <int DIM, int MODE>
 struct Raw {
   static constexpr int N = MODE == SRC_PLAIN ? DIM + 2 : 8;
@@ -230,13 +237,13 @@ struct Raw {
 };
 
 template <int DIM, int MODE>
-__device__ __forceinline__ void load_raw(const float* const* s, long c, Raw<DIM, MODE>& r) {
+__device__ __forceinline__ void load_raw(const float* const* s, unsigned c, Raw<DIM, MODE>& r) {
   if (MODE == SRC_PLAIN) {
 #pragma unroll
-    for (int v = 0; v < DIM + 2; ++v) r.f[v] = s[v][c];
+    for (int v = 0; v < DIM + 2; ++v) r.f[v] = ld32(s[v], c);
   } else {
 #pragma unroll
-    for (int v = 0; v < 8; ++v) r.f[v] = (DIM == 2 && v == 4) ? 0.f : s[v][c];
+    for (int v = 0; v < 8; ++v) r.f[v] = (DIM == 2 && v == 4) ? 0.f : ld32(s[v], c);
   }
 }
 
@@ -265,7 +272,7 @@ __device__ __forceinline__ void derive_w(const Raw<DIM, MODE>& r, float* w) {
 }
 
 template <int DIM, int MODE>
-__device__ __forceinline__ void load_w(const float* const* s, long c, float* w) {
+__device__ __forceinline__ void load_w(const float* const* s, unsigned c, float* w) {
   Raw<DIM, MODE> r;
   load_raw<DIM, MODE>(s, c, r);
   derive_w<DIM, MODE>(r, w);
@@ -349,16 +356,18 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
   const int ita = rest % a.nta;
   const int icx = rest / a.nta;
   const int a0 = ita * TA, b0 = itb * TB;
-  const long sa = a.nb, sx = (long)a.na * a.nb;
+  // 32-bit cell indices (a slot has < 2^30 cells): global loads then take a
+  // scalar base and a 32-bit vector offset
+  const unsigned sa = a.nb, sx = (unsigned)a.na * a.nb;
   const int ga = wrapi(a0 + ta, a.na), gb = wrapi(b0 + tb, a.nb);
-  const long col = (long)ga * sa + gb;
+  const unsigned col = (unsigned)ga * sa + gb;
   const bool active = owner && (a0 + ta < a.na) && (b0 + tb < a.nb);
   const float coef = ph.inv_dx * (a.dt_scale * C->dt);
 
   // this thread's halo cell (threads NT-NH .. NT-1), fixed for all planes
   const int h = tid - (NT - NH);
   int hr = 0, hc = 0;
-  long hoff = 0;
+  unsigned hoff = 0;
   if (h >= 0) {
     if (h < 4 * TB) {
       const int q = h / TB;
@@ -370,7 +379,7 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
       const int q = j & 3;
       hc = q < 2 ? q - 2 : TB + q - 2;
     }
-    hoff = (long)wrapi(a0 + hr, a.na) * sa + wrapi(b0 + hc, a.nb);
+    hoff = (unsigned)wrapi(a0 + hr, a.na) * sa + wrapi(b0 + hc, a.nb);
   }
 
   int x0 = 0, x1 = 1;
@@ -382,10 +391,10 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
   if (owner) {
     if (DIM == 3) {
       float qm2[NV], qm1[NV], wl[NV], wr[NV];
-      load_w<DIM, SRC>(src, (long)wrapi(x0 - 2, a.nx) * sx + col, qm2);
-      load_w<DIM, SRC>(src, (long)wrapi(x0 - 1, a.nx) * sx + col, qm1);
-      load_w<DIM, SRC>(src, (long)x0 * sx + col, q0);
-      load_w<DIM, SRC>(src, (long)wrapi(x0 + 1, a.nx) * sx + col, q1);
+      load_w<DIM, SRC>(src, (unsigned)wrapi(x0 - 2, a.nx) * sx + col, qm2);
+      load_w<DIM, SRC>(src, (unsigned)wrapi(x0 - 1, a.nx) * sx + col, qm1);
+      load_w<DIM, SRC>(src, (unsigned)x0 * sx + col, q0);
+      load_w<DIM, SRC>(src, (unsigned)wrapi(x0 + 1, a.nx) * sx + col, q1);
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
         d0[v] = half_slope(qm1[v], q0[v], q1[v]);
@@ -402,23 +411,23 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
   // registers across the previous plane's flux phase)
   Raw<DIM, SRC> rq, rh;  // own column at x+2, halo cell at x
   Raw<DIM, W0M> r0;      // Heun centre at x
-  if (owner && DIM == 3) load_raw<DIM, SRC>(src, (long)wrapi(x0 + 2, a.nx) * sx + col, rq);
-  if (h >= 0) load_raw<DIM, SRC>(src, (DIM == 3 ? (long)x0 * sx : 0) + hoff, rh);
+  if (owner && DIM == 3) load_raw<DIM, SRC>(src, (unsigned)wrapi(x0 + 2, a.nx) * sx + col, rq);
+  if (h >= 0) load_raw<DIM, SRC>(src, (DIM == 3 ? (unsigned)x0 * sx : 0u) + hoff, rh);
   // the 8-field energy-fix centre is not prefetched (register pressure)
   constexpr bool PRE0 = HEUN && W0M == SRC_PLAIN;
-  if (PRE0 && active) load_raw<DIM, W0M>(w0p, (DIM == 3 ? (long)x0 * sx : 0) + col, r0);
+  if (PRE0 && active) load_raw<DIM, W0M>(w0p, (DIM == 3 ? (unsigned)x0 * sx : 0u) + col, r0);
 
   float mx[3] = {0.f, 0.f, 0.f};
   for (int x = x0; x < x1; ++x) {
-    const long plane = DIM == 3 ? (long)x * sx : 0;
-    const long nplane = DIM == 3 ? (long)(x + 1) * sx : 0;
+    const unsigned plane = DIM == 3 ? (unsigned)x * sx : 0u;
+    const unsigned nplane = DIM == 3 ? (unsigned)(x + 1) * sx : 0u;
     const bool more = x + 1 < x1;
     float Fn[NV], q2[NV], hw[NV];
     Raw<DIM, W0M> r0c = r0;
     if (owner && DIM == 3) derive_w<DIM, SRC>(rq, q2);
     if (h >= 0) derive_w<DIM, SRC>(rh, hw);
     if (more) {
-      if (owner && DIM == 3) load_raw<DIM, SRC>(src, (long)wrapi(x + 3, a.nx) * sx + col, rq);
+      if (owner && DIM == 3) load_raw<DIM, SRC>(src, (unsigned)wrapi(x + 3, a.nx) * sx + col, rq);
       if (h >= 0) load_raw<DIM, SRC>(src, nplane + hoff, rh);
       if (PRE0 && active) load_raw<DIM, W0M>(w0p, nplane + col, r0);
     }
@@ -475,7 +484,7 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
     }
     __syncthreads();
     if (active) {
-      const long cell = plane + col;
+      const unsigned cell = plane + col;
       float U[NV], W1[NV];
       cons<DIM>(q0, U, ph);
       if (DIM == 3) {
@@ -501,7 +510,7 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
         prim<DIM>(U, W1, ph);
       }
 #pragma unroll
-      for (int v = 0; v < NV; ++v) outp[v][cell] = W1[v];
+      for (int v = 0; v < NV; ++v) st32(outp[v], cell, W1[v]);
       if (a.want_cfl) {
         const float cs = fsqrt(ph.gamma * W1[P] * frcp(W1[0]));
 #pragma unroll
@@ -512,7 +521,7 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
       }
       if (a.copy_b) {
 #pragma unroll
-        for (int k = 0; k < 3; ++k) a.bdst[k][so + cell] = a.selb[par][k][so + cell];
+        for (int k = 0; k < 3; ++k) st32(a.bdst[k] + so, cell, ld32(a.selb[par][k] + so, cell));
       }
     }
     if (DIM == 3 && owner) {
@@ -1207,6 +1216,8 @@ amx_status validate(const amx_fv_config* c) {
     return fail(AMX_ERR_INVALID, "max_fixed_point_iterations out of range%s");
   if (!(c->grid_spacing > 0)) return fail(AMX_ERR_INVALID, "grid_spacing must be positive%s");
   if (!(c->gamma > 1.0f)) return fail(AMX_ERR_INVALID, "gamma must exceed 1%s");
+  if ((long)c->nx * c->ny * (c->dim == 3 ? c->nz : 1) >= (1L << 30))
+    return fail(AMX_ERR_INVALID, "a grid of 2^30 cells or more is not supported (32-bit cell indices)%s");
   return AMX_OK;
 }
 
