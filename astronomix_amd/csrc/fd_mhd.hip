@@ -49,18 +49,29 @@ struct Grid {
 
 __device__ __forceinline__ long cell(const Grid& g, int x, int y, int z) { return (long)x * g.sx + (long)y * g.sy + z; }
 
-// neighbour of flat cell c along axis by s (periodic)
-__device__ __forceinline__ long nbr(const Grid& g, long c, int axis, int s) {
-  if (axis == 2) {
-    const int z = (int)(c % g.nz);
-    return c - z + wrapi(z + s, g.nz);
-  }
-  if (axis == 1) {
-    const int y = (int)((c / g.sy) % g.ny);
-    return c + (long)(wrapi(y + s, g.ny) - y) * g.sy;
-  }
-  const int x = (int)(c / g.sx);
-  return c + (long)(wrapi(x + s, g.nx) - x) * g.sx;
+// Cell index with its coordinates: computed once per cell with 32-bit
+// arithmetic (grids have < 2^30 cells), so that stencil neighbours need no
+// division.
+struct Pos {
+  unsigned c;
+  int x, y, z;
+};
+
+__device__ __forceinline__ Pos pos_of(const Grid& g, unsigned c) {
+  Pos p;
+  p.c = c;
+  const unsigned t = c / (unsigned)g.nz;
+  p.z = (int)(c - t * (unsigned)g.nz);
+  p.x = (int)(t / (unsigned)g.ny);
+  p.y = (int)(t - (unsigned)p.x * (unsigned)g.ny);
+  return p;
+}
+
+// neighbour of p along axis by s (periodic)
+__device__ __forceinline__ unsigned nb(const Grid& g, const Pos& p, int axis, int s) {
+  if (axis == 2) return p.c - p.z + wrapi(p.z + s, g.nz);
+  if (axis == 1) return p.c + (unsigned)((wrapi(p.y + s, g.ny) - p.y) * g.nz);
+  return p.c + (unsigned)((wrapi(p.x + s, g.nx) - p.x)) * (unsigned)g.sx;
 }
 
 __device__ __forceinline__ float dss(float x) { return fsqrt(fmaxf(x, 1e-12f)); }
@@ -502,30 +513,32 @@ __global__ __launch_bounds__(NTH) void k_weno(const WenoArgs a) {
 }
 
 // ---------------------------------------------------------------- stencils
-__device__ __forceinline__ float c2f(const Grid& g, const float* a, long c, int ax) {
-  return (((-a[nbr(g, c, ax, -1)] + 9.f * a[c]) + 9.f * a[nbr(g, c, ax, 1)]) - a[nbr(g, c, ax, 2)]) * (1.f / 16.f);
+__device__ __forceinline__ float c2f(const Grid& g, const float* a, const Pos& p, int ax) {
+  return (((-a[nb(g, p, ax, -1)] + 9.f * a[p.c]) + 9.f * a[nb(g, p, ax, 1)]) - a[nb(g, p, ax, 2)]) * (1.f / 16.f);
 }
 
-__device__ __forceinline__ float f2c(const Grid& g, const float* f, long c, int ax) {
-  return (((((3.f * f[nbr(g, c, ax, -3)] - 25.f * f[nbr(g, c, ax, -2)]) + 150.f * f[nbr(g, c, ax, -1)]) + 150.f * f[c]) -
-           25.f * f[nbr(g, c, ax, 1)]) +
-          3.f * f[nbr(g, c, ax, 2)]) *
+__device__ __forceinline__ float f2c(const Grid& g, const float* f, const Pos& p, int ax) {
+  return (((((3.f * f[nb(g, p, ax, -3)] - 25.f * f[nb(g, p, ax, -2)]) + 150.f * f[nb(g, p, ax, -1)]) + 150.f * f[p.c]) -
+           25.f * f[nb(g, p, ax, 1)]) +
+          3.f * f[nb(g, p, ax, 2)]) *
          (1.f / 256.f);
 }
 
-__device__ __forceinline__ float d6(const Grid& g, const float* f, long c, int ax) {
+__device__ __forceinline__ float d6(const Grid& g, const float* f, const Pos& p, int ax) {
   const float c1 = 75.f / 64.f, c2 = -25.f / 384.f, c3 = 3.f / 640.f;
-  return (c1 * (f[c] - f[nbr(g, c, ax, -1)]) + c2 * (f[nbr(g, c, ax, 1)] - f[nbr(g, c, ax, -2)])) +
-         c3 * (f[nbr(g, c, ax, 2)] - f[nbr(g, c, ax, -3)]);
+  return (c1 * (f[p.c] - f[nb(g, p, ax, -1)]) + c2 * (f[nb(g, p, ax, 1)] - f[nb(g, p, ax, -2)])) +
+         c3 * (f[nb(g, p, ax, 2)] - f[nb(g, p, ax, -3)]);
 }
 
-__device__ __forceinline__ float p2a(const Grid& g, const float* q, long c, int A, int B) {
-  const float sa = ((q[nbr(g, c, A, -1)] - 2.f * q[c]) + q[nbr(g, c, A, 1)]) * (1.f / 24.f);
-  const float sb = ((q[nbr(g, c, B, -1)] - 2.f * q[c]) + q[nbr(g, c, B, 1)]) * (1.f / 24.f);
-  return (q[c] + sa) + sb;
+__device__ __forceinline__ float p2a(const Grid& g, const float* q, const Pos& p, int A, int B) {
+  const float sa = ((q[nb(g, p, A, -1)] - 2.f * q[p.c]) + q[nb(g, p, A, 1)]) * (1.f / 24.f);
+  const float sb = ((q[nb(g, p, B, -1)] - 2.f * q[p.c]) + q[nb(g, p, B, 1)]) * (1.f / 24.f);
+  return (q[p.c] + sa) + sb;
 }
 
-#define FD_LOOP(g) for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < (g).nc; c += (long)gridDim.x * blockDim.x)
+#define FD_LOOP(g)                                                                     \
+  for (unsigned c = blockIdx.x * blockDim.x + threadIdx.x; c < (unsigned)(g).nc; \
+       c += gridDim.x * blockDim.x)
 
 // _enforce_positivity on the stage state (in place, optional) and
 // update_cell_center_fields into qt = (E, B_x, B_y, B_z)
@@ -542,6 +555,8 @@ __global__ void k_prep(const PrepArgs a) {
   const Grid g = a.g;
   const Phys ph = a.ph;
   FD_LOOP(g) {
+    const Pos P = pos_of(g, c);
+    (void)P;
     float rho = a.q[0][c], E = a.q[4][c];
     const float Bx = a.q[5][c], By = a.q[6][c], Bz = a.q[7][c];
     const float b2 = (Bx * Bx + By * By) + Bz * Bz;
@@ -555,7 +570,7 @@ __global__ void k_prep(const PrepArgs a) {
       a.q[0][c] = rho;
       a.q[4][c] = E;
     }
-    const float X = f2c(g, a.b[0], c, 0), Y = f2c(g, a.b[1], c, 1), Z = f2c(g, a.b[2], c, 2);
+    const float X = f2c(g, a.b[0], P, 0), Y = f2c(g, a.b[1], P, 1), Z = f2c(g, a.b[2], P, 2);
     const float nb2 = (X * X + Y * Y) + Z * Z;
     a.qt[0][c] = E + 0.5f * (nb2 - b2);
     a.qt[1][c] = X;
@@ -579,9 +594,11 @@ struct EdgeArgs {
 __global__ void k_ct_edge(const EdgeArgs a) {
   const Grid g = a.g;
   FD_LOOP(g) {
-    a.O[2][c] = c2f(g, a.fxy, c, 0) - c2f(g, a.fyx, c, 1);
-    a.O[0][c] = c2f(g, a.fyz, c, 1) - c2f(g, a.fzy, c, 2);
-    a.O[1][c] = c2f(g, a.fzx, c, 2) - c2f(g, a.fxz, c, 0);
+    const Pos P = pos_of(g, c);
+    (void)P;
+    a.O[2][c] = c2f(g, a.fxy, P, 0) - c2f(g, a.fyx, P, 1);
+    a.O[0][c] = c2f(g, a.fyz, P, 1) - c2f(g, a.fzy, P, 2);
+    a.O[1][c] = c2f(g, a.fzx, P, 2) - c2f(g, a.fxz, P, 0);
   }
 }
 
@@ -594,9 +611,11 @@ struct AvgArgs {
 __global__ void k_ct_avg(const AvgArgs a) {
   const Grid g = a.g;
   FD_LOOP(g) {
-    a.Ob[2][c] = p2a(g, a.O[2], c, 0, 1);
-    a.Ob[0][c] = p2a(g, a.O[0], c, 1, 2);
-    a.Ob[1][c] = p2a(g, a.O[1], c, 0, 2);
+    const Pos P = pos_of(g, c);
+    (void)P;
+    a.Ob[2][c] = p2a(g, a.O[2], P, 0, 1);
+    a.Ob[0][c] = p2a(g, a.O[0], P, 1, 2);
+    a.Ob[1][c] = p2a(g, a.O[1], P, 0, 2);
   }
 }
 
@@ -622,9 +641,11 @@ __global__ void k_face(const FaceArgs a) {
   const Grid g = a.g;
   const float dtdx = (a.k.k2 * (*a.dtp)) * a.inv_dx;
   FD_LOOP(g) {
-    const float r[3] = {(-dtdx) * d6(g, a.Ob[2], c, 1) + dtdx * d6(g, a.Ob[1], c, 2),
-                        (-dtdx) * d6(g, a.Ob[0], c, 2) + dtdx * d6(g, a.Ob[2], c, 0),
-                        (-dtdx) * d6(g, a.Ob[1], c, 0) + dtdx * d6(g, a.Ob[0], c, 1)};
+    const Pos P = pos_of(g, c);
+    (void)P;
+    const float r[3] = {(-dtdx) * d6(g, a.Ob[2], P, 1) + dtdx * d6(g, a.Ob[1], P, 2),
+                        (-dtdx) * d6(g, a.Ob[0], P, 2) + dtdx * d6(g, a.Ob[2], P, 0),
+                        (-dtdx) * d6(g, a.Ob[1], P, 0) + dtdx * d6(g, a.Ob[0], P, 1)};
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
       if (a.k.final_stage) {
@@ -656,7 +677,9 @@ __global__ void k_qupd(const QupdArgs a) {
   const Grid g = a.g;
   const float dtdx = (a.k.k2 * (*a.dtp)) * a.inv_dx;
   FD_LOOP(g) {
-    const long xm = nbr(g, c, 0, -1), ym = nbr(g, c, 1, -1), zm = nbr(g, c, 2, -1);
+    const Pos P = pos_of(g, c);
+    (void)P;
+    const unsigned xm = nb(g, P, 0, -1), ym = nb(g, P, 1, -1), zm = nb(g, P, 2, -1);
 #pragma unroll
     for (int v = 0; v < 8; ++v) {
       const float rhs = (-dtdx) * (((a.Fx[v][c] - a.Fx[v][xm]) + (a.Fy[v][c] - a.Fy[v][ym])) + (a.Fz[v][c] - a.Fz[v][zm]));
@@ -689,6 +712,8 @@ __global__ void k_init(const InitArgs a) {
   const Grid g = a.g;
   const Phys ph = a.ph;
   FD_LOOP(g) {
+    const Pos P = pos_of(g, c);
+    (void)P;
     const float rho = a.w[0][c];
     const float vx = a.w[1][c], vy = a.w[2][c], vz = a.w[3][c];
     const float Bx = a.w[5][c], By = a.w[6][c], Bz = a.w[7][c];
@@ -724,10 +749,12 @@ __global__ void k_finish(const FinishArgs a) {
   const Grid g = a.g;
   const Phys ph = a.ph;
   FD_LOOP(g) {
+    const Pos P = pos_of(g, c);
+    (void)P;
     float rho = a.qf[0][c], E = a.qf[4][c];
     const float m[3] = {a.qf[1][c], a.qf[2][c], a.qf[3][c]};
     const float ob2 = (a.qf[5][c] * a.qf[5][c] + a.qf[6][c] * a.qf[6][c]) + a.qf[7][c] * a.qf[7][c];
-    const float X = f2c(g, a.bf[0], c, 0), Y = f2c(g, a.bf[1], c, 1), Z = f2c(g, a.bf[2], c, 2);
+    const float X = f2c(g, a.bf[0], P, 0), Y = f2c(g, a.bf[1], P, 1), Z = f2c(g, a.bf[2], P, 2);
     const float b2 = (X * X + Y * Y) + Z * Z;
     E = E + 0.5f * (b2 - ob2);
     if (ph.positivity) {
@@ -850,6 +877,8 @@ amx_status fd_validate(const amx_fd_config* c) {
   if (!c) return amx_fail(AMX_ERR_INVALID, "null config%s");
   if (c->nx < 6 || c->ny < 6 || c->nz < 6) return amx_fail(AMX_ERR_INVALID, "FD grid needs >= 6 cells per axis%s");
   if (!(c->grid_spacing > 0)) return amx_fail(AMX_ERR_INVALID, "grid_spacing must be positive%s");
+  if ((long)c->nx * c->ny * c->nz >= (1L << 30))
+    return amx_fail(AMX_ERR_INVALID, "a grid of 2^30 cells or more is not supported (32-bit cell indices)%s");
   return AMX_OK;
 }
 
@@ -1119,6 +1148,8 @@ __global__ void k_cfl(const CflArgs a) {
   const Phys ph = a.ph;
   float mx[3] = {0.f, 0.f, 0.f};
   FD_LOOP(g) {
+    const Pos P = pos_of(g, c);
+    (void)P;
     const float rho = a.w[0][c];
     const float vx = a.w[1][c], vy = a.w[2][c], vz = a.w[3][c];
     const float Bx = a.w[5][c], By = a.w[6][c], Bz = a.w[7][c];
@@ -1269,8 +1300,10 @@ struct C2fArgs {
 __global__ void k_interface_fields(const C2fArgs a) {
   const Grid g = a.g;
   FD_LOOP(g) {
+    const Pos P = pos_of(g, c);
+    (void)P;
 #pragma unroll
-    for (int d = 0; d < 3; ++d) a.b[d][c] = c2f(g, a.B[d], c, d);
+    for (int d = 0; d < 3; ++d) a.b[d][c] = c2f(g, a.B[d], P, d);
   }
 }
 
