@@ -40,7 +40,7 @@ namespace amx {
 
 constexpr int TB = 64;      // tile width along the contiguous axis (one wavefront)
 #ifndef AMX_STAGE_WAVES
-#define AMX_STAGE_WAVES 5   // >= 5 waves/SIMD keeps two 9-wave stage blocks per CU
+#define AMX_STAGE_WAVES 4   // two 8-wave stage blocks per CU: 4 waves/SIMD, <= 128 VGPRs
 #endif
 
 // Device-resident step control: step size, time, fixed-point bookkeeping and
