@@ -3,7 +3,9 @@ checker of the HIP kernels.
 
 * the reference's one golden output (tests/mhd_tests/final_state_orszag_tang.npz,
   float64 Orszag-Tang 200^2 at t = 3 in a 2 pi box), compared at the gap the
-  current reference source leaves to it (see DESIGN.md, "golden gap");
+  current reference source leaves to it: the file predates that source, and
+  the restatement of the source agrees with it to 0.7e-3..1.5e-3 relative L1
+  whatever the step size, tolerance or precision;
 * closed forms: minmod, Euler-flux consistency of HLL/HLLC, supersonic
   upwinding, the HLLC stationary contact, the CFL step of a uniform state;
 * invariants: uniform states are fixed points, periodic flux form conserves
@@ -66,7 +68,7 @@ def test_oracle_against_reference_golden(oracle):
     assert t == pytest.approx(3.0, abs=1e-12)
     err = rel_l1(out, g)
     # the golden predates the current reference source: 0.7e-3..1.5e-3 relative
-    # L1, independent of dt and of float64 round-off ("golden gap")
+    # L1, independent of dt and of float64 round-off
     assert err[:6].max() < 2.5e-3, err
     # exact properties of the golden the restatement must share
     assert abs(out[0].mean() - 25.0 / 9.0) < 1e-12
