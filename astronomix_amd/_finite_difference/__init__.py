@@ -16,7 +16,6 @@ States are float32 torch tensors on the GPU, (11, nx, ny, nz) =
 
 from __future__ import annotations
 
-import time
 
 import numpy as np
 
@@ -152,11 +151,13 @@ def initialize_interface_fields(magnetic_field_x, magnetic_field_y, magnetic_fie
     return out[0], out[1], out[2]
 
 
-def time_integration_fd(primitive_state, config, params, registered_variables):
+def time_integration_fd(primitive_state, config, params, registered_variables, snapshot_callable=None):
     """FD branch of time_integration: device-resident SSPRK(5,4) loop with
     the CFL step (clamped to params.dt_max) or t_end / num_timesteps, and
-    the exact-end clamp."""
+    the exact-end clamp; SnapshotData / snapshot callbacks as in the FV path
+    (max |div B| over the face fields)."""
     import torch
+    from astronomix_amd.time_stepping._snapshots import exact_end_time, run_loop, to_host
     require_supported(config, registered_variables)
     struct = None
     if config.state_struct:
@@ -174,7 +175,7 @@ def time_integration_fd(primitive_state, config, params, registered_variables):
             else primitive_state.contiguous().clone()
     ctx = _native.fd_context(native_config(state.shape, config, params))
     run = _native.AmxFvRun(t_end=float(params.t_end), fixed_timestep=int(bool(config.fixed_timestep)),
-                           num_timesteps=int(config.num_timesteps), exact_end_time=int(bool(config.exact_end_time)),
+                           num_timesteps=int(config.num_timesteps), exact_end_time=int(exact_end_time(config)),
                            max_steps=-1)
     if config.memory_analysis:
         import ctypes
@@ -183,20 +184,20 @@ def time_integration_fd(primitive_state, config, params, registered_variables):
         print("=== device memory PER DEVICE ===")
         print(f"State size: {state.numel() * 4 / 1024**2:.2f} MB")
         print(f"Workspace size: {nbytes.value / 1024**2:.2f} MB")
-    t0 = time.perf_counter()
     if config.print_elapsed_time:
         print("Starting simulation...")
-    stats = ctx.time_integration(state, run)
+    stats, snaps, seconds = run_loop(ctx, state, run, config, params, registered_variables, snapshot_callable)
     if config.print_elapsed_time:
-        torch.cuda.synchronize()
         print("Simulation finished!")
-        print(f"Time elapsed: {time.perf_counter() - t0:.2f} seconds")
+        print(f"Time elapsed: {seconds:.2f} seconds")
         print(f"Number of iterations: {stats.steps}")
     if config.runtime_debugging:
         if not torch.isfinite(state).all():
             raise FloatingPointError("non-finite values in the final state")
         if (state[4] < 0).any() or (state[0] < 0).any():
             raise FloatingPointError("negative density or pressure in the final state")
+    if snaps is not None:
+        return to_host(snaps) if host_numpy else snaps
     out = state.cpu().numpy() if host_numpy else state
     if struct is not None:
         return struct._replace(primitive_state=out)

@@ -91,6 +91,54 @@ class AmxFdConfig(ctypes.Structure):
     ]
 
 
+AMX_DIV_NONE, AMX_DIV_CENTRAL, AMX_DIV_FACES = 0, 1, 2
+
+
+class AmxDiagDesc(ctypes.Structure):
+    _fields_ = [
+        ("num_vars", ctypes.c_int),
+        ("dim", ctypes.c_int),
+        ("nx", ctypes.c_int),
+        ("ny", ctypes.c_int),
+        ("nz", ctypes.c_int),
+        ("rho_row", ctypes.c_int),
+        ("vel_row", ctypes.c_int),
+        ("p_row", ctypes.c_int),
+        ("mag_row", ctypes.c_int),
+        ("div_mode", ctypes.c_int),
+        ("gamma", ctypes.c_float),
+        ("grid_spacing", ctypes.c_double),
+        ("box_size", ctypes.c_double),
+    ]
+
+
+# SnapshotData fields recorded per snapshot, in amx_snapshot_out order
+SNAPSHOT_FIELDS = ("time_points", "states", "total_mass", "total_energy", "internal_energy", "kinetic_energy",
+                   "radial_momentum", "magnetic_divergence")
+
+
+class AmxSnapshotOut(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_void_p) for name in SNAPSHOT_FIELDS] + [
+        ("current_checkpoint", ctypes.c_int),
+        ("num_iterations", ctypes.c_long),
+    ]
+
+
+SNAPSHOT_CALLBACK = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_float, ctypes.c_int, ctypes.c_void_p)
+
+
+class AmxSnapshotSpec(ctypes.Structure):
+    _fields_ = [
+        ("num_snapshots", ctypes.c_int),
+        ("use_timepoints", ctypes.c_int),
+        ("timepoints", ctypes.POINTER(ctypes.c_float)),
+        ("num_timepoints", ctypes.c_int),
+        ("box_size", ctypes.c_double),
+        ("callback", SNAPSHOT_CALLBACK),
+        ("user", ctypes.c_void_p),
+    ]
+
+
 # Every symbol include/amx.h declares, with its ctypes signature.
 _P = ctypes.c_void_p
 SIGNATURES = {
@@ -124,6 +172,17 @@ SIGNATURES = {
     "amx_fd_interface_fields": (ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P]),
     "amx_fd_set_timing": (ctypes.c_int, [_P, ctypes.c_int]),
     "amx_fd_get_timing": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_long)]),
+    "amx_snapshot_workspace_bytes": (ctypes.c_size_t, []),
+    "amx_snapshot_record": (ctypes.c_int, [ctypes.POINTER(AmxDiagDesc), _P, ctypes.c_float,
+                                           ctypes.POINTER(AmxSnapshotOut), ctypes.c_int, _P, _P]),
+    "amx_fv_time_integration_snapshots": (ctypes.c_int, [_P, _P, ctypes.POINTER(AmxFvRun),
+                                                         ctypes.POINTER(AmxSnapshotSpec),
+                                                         ctypes.POINTER(AmxSnapshotOut),
+                                                         ctypes.POINTER(AmxFvStats), _P]),
+    "amx_fd_time_integration_snapshots": (ctypes.c_int, [_P, _P, ctypes.POINTER(AmxFvRun),
+                                                         ctypes.POINTER(AmxSnapshotSpec),
+                                                         ctypes.POINTER(AmxSnapshotOut),
+                                                         ctypes.POINTER(AmxFvStats), _P]),
     "amx_copy_f32": (ctypes.c_int, [_P, _P, ctypes.c_size_t, _P]),
     "amx_status_string": (ctypes.c_char_p, [ctypes.c_int]),
     "amx_last_error": (ctypes.c_char_p, []),
@@ -229,6 +288,12 @@ class FvContext:
         st = AmxFvStats()
         check(self.lib.amx_fv_time_integration(self.handle, ctypes.c_void_p(state.data_ptr()), ctypes.byref(run),
                                                ctypes.byref(st), self._stream(stream)), "amx_fv_time_integration")
+        return st
+
+    def time_integration_snapshots(self, state, run: AmxFvRun, snapshots: "SnapshotRun", stream=None) -> AmxFvStats:
+        st = AmxFvStats()
+        snapshots.invoke(self.lib.amx_fv_time_integration_snapshots, self.handle, state, run, st,
+                         self._stream(stream), "amx_fv_time_integration_snapshots")
         return st
 
 
@@ -353,6 +418,86 @@ class FdContext:
         check(self.lib.amx_fd_time_integration(self.handle, ctypes.c_void_p(state.data_ptr()), ctypes.byref(run),
                                                ctypes.byref(st), self._stream(stream)), "amx_fd_time_integration")
         return st
+
+    def time_integration_snapshots(self, state, run: AmxFvRun, snapshots: "SnapshotRun", stream=None) -> AmxFvStats:
+        st = AmxFvStats()
+        snapshots.invoke(self.lib.amx_fd_time_integration_snapshots, self.handle, state, run, st,
+                         self._stream(stream), "amx_fd_time_integration_snapshots")
+        return st
+
+
+class SnapshotRun:
+    """Host side of one snapshot run: the amx_snapshot_spec, the device
+    arrays of amx_snapshot_out (torch tensors, kept alive here) and, in
+    callback mode, the ctypes trampoline. An exception raised by the Python
+    callback stops the run and is re-raised after the C call returns."""
+
+    def __init__(self, num_snapshots: int, box_size: float, arrays: dict = None, timepoints=None,
+                 callback=None):
+        self.arrays = dict(arrays or {})
+        self.out = AmxSnapshotOut()
+        for name in SNAPSHOT_FIELDS:
+            t = self.arrays.get(name)
+            setattr(self.out, name, ctypes.c_void_p(t.data_ptr()) if t is not None else None)
+        self._tp = None
+        n_tp = 0
+        if timepoints is not None:
+            vals = [float(v) for v in timepoints]
+            n_tp = len(vals)
+            self._tp = (ctypes.c_float * max(n_tp, 1))(*vals)
+        self._error = None
+        self._cb = None
+        if callback is not None:
+            def trampoline(user, t, checkpoint, state_ptr):
+                try:
+                    callback(float(t), int(checkpoint), int(state_ptr or 0))
+                    return 0
+                except BaseException as e:  # noqa: BLE001 -- re-raised after the C call
+                    self._error = e
+                    return 1
+            self._cb = SNAPSHOT_CALLBACK(trampoline)
+        self.spec = AmxSnapshotSpec(
+            num_snapshots=int(num_snapshots), use_timepoints=int(timepoints is not None),
+            timepoints=ctypes.cast(self._tp, ctypes.POINTER(ctypes.c_float)) if self._tp is not None else None,
+            num_timepoints=n_tp, box_size=float(box_size),
+            callback=self._cb if self._cb is not None else SNAPSHOT_CALLBACK(), user=None)
+
+    def invoke(self, fn, handle, state, run, stats, stream, what):
+        status = fn(handle, ctypes.c_void_p(state.data_ptr()), ctypes.byref(run), ctypes.byref(self.spec),
+                    ctypes.byref(self.out), ctypes.byref(stats), stream)
+        if self._error is not None:
+            err, self._error = self._error, None
+            raise err
+        check(status, what)
+
+    @property
+    def current_checkpoint(self) -> int:
+        return int(self.out.current_checkpoint)
+
+    @property
+    def num_iterations(self) -> int:
+        return int(self.out.num_iterations)
+
+
+def snapshot_workspace_bytes() -> int:
+    return int(load_library().amx_snapshot_workspace_bytes())
+
+
+def snapshot_record(desc: AmxDiagDesc, state, t: float, arrays: dict, index: int = 0, work=None, stream=None):
+    """One snapshot row of `state` into the device arrays named in `arrays`
+    (SNAPSHOT_FIELDS keys, float32 tensors) at `index`."""
+    import torch
+    lib = load_library()
+    if work is None:
+        work = torch.empty(snapshot_workspace_bytes(), dtype=torch.uint8, device=state.device)
+    out = AmxSnapshotOut()
+    for name in SNAPSHOT_FIELDS:
+        a = arrays.get(name)
+        setattr(out, name, ctypes.c_void_p(a.data_ptr()) if a is not None else None)
+    s = stream if stream is not None else torch.cuda.current_stream()
+    check(lib.amx_snapshot_record(ctypes.byref(desc), ctypes.c_void_p(state.data_ptr()), float(t), ctypes.byref(out),
+                                  int(index), ctypes.c_void_p(work.data_ptr()), ctypes.c_void_p(s.cuda_stream)),
+          "amx_snapshot_record")
 
 
 def copy_f32(dst, src, stream=None):

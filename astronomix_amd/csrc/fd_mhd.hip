@@ -28,6 +28,7 @@
 
 #include "amx.h"
 #include "common.h"
+#include "snapshots.h"
 
 namespace amx {
 namespace fd {
@@ -799,6 +800,8 @@ struct Ctl {
 struct StepParams {
   float t_end, C, dx, dt_max;
   int fixed, num_timesteps, exact_end, use_cfl;
+  int clamp_tp = 0;  // clamp dt to tp - t (snapshot time points, time_integration.py:671-674)
+  float tp = 0.f;
 };
 
 // dt = min(C dx / (lx + ly + lz), dt_max) (_timestep_estimator.py(FD):89-90),
@@ -814,6 +817,10 @@ __global__ void k_step_begin(Ctl* c, StepParams p) {
     dt = p.dt_max < dt ? p.dt_max : dt;
   } else {
     dt = c->dt;
+  }
+  if (p.clamp_tp) {
+    const float rem = p.tp - c->t;
+    dt = rem < dt ? rem : dt;
   }
   if (p.exact_end) {
     const float rem = p.t_end - c->t;
@@ -867,6 +874,7 @@ struct amx_fd_ctx {
   int ev_used;
   double weno_ms;
   long weno_launches;
+  void* snapwork;  // snapshot diagnostics partials (allocated on first use)
 };
 
 namespace {
@@ -1125,6 +1133,7 @@ extern "C" amx_status amx_fd_destroy(amx_fd_ctx* c) {
   (void)hipFree(c->ws);
   (void)hipFree(c->ctl);
   (void)hipHostFree(c->hctl);
+  if (c->snapwork) (void)hipFree(c->snapwork);
   for (auto e : c->ev) (void)hipEventDestroy(e);
   delete c;
   return AMX_OK;
@@ -1227,14 +1236,37 @@ extern "C" amx_status amx_fd_evolve_state(amx_fd_ctx* c, float* state, const flo
   return fd_step(c, state, st);
 }
 
-extern "C" amx_status amx_fd_time_integration(amx_fd_ctx* c, float* state, const amx_fv_run* run, amx_fv_stats* stats,
-                                              void* stream) {
-  if (!c || !state || !run) return amx_fail(AMX_ERR_INVALID, "null argument%s");
-  if (run->fixed_timestep && run->num_timesteps <= 0) return amx_fail(AMX_ERR_INVALID, "num_timesteps must be positive%s");
-  hipStream_t st = (hipStream_t)stream;
+namespace {
+
+// Layout of an FD state (registered_variables.py:188-198) for the snapshot
+// diagnostics: max |div B| is taken over the face fields.
+amx_diag_desc fd_diag_desc(const amx_fd_ctx* c, double box_size) {
+  amx_diag_desc d;
+  d.num_vars = 11;
+  d.dim = 3;
+  d.nx = c->cfg.nx;
+  d.ny = c->cfg.ny;
+  d.nz = c->cfg.nz;
+  d.rho_row = 0;
+  d.vel_row = 1;
+  d.p_row = 4;
+  d.mag_row = 8;
+  d.div_mode = AMX_DIV_FACES;
+  d.gamma = c->cfg.gamma;
+  d.grid_spacing = c->cfg.grid_spacing;
+  d.box_size = box_size;
+  return d;
+}
+
+// The FD branch of the step loop (time_integration.py:629-801), with the
+// snapshot rows of time_integration.py:539-610, 736-745 when snp is given.
+amx_status fd_loop(amx_fd_ctx* c, float* state, const amx_fv_run* run, amx::snap::Driver* snp, amx_fv_stats* stats,
+                   hipStream_t st) {
   hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, st, c->ctl);
   AMX_HIPCHK(hipGetLastError());
-  const StepParams sp = fd_params(c, run, 1);
+  StepParams sp = fd_params(c, run, 1);
+  // the end-time clamp is off with snapshot time points (time_integration.py:677)
+  if (snp && snp->spec->use_timepoints) sp.exact_end = 0;
   long steps = 0;
   float t = 0.f;
   for (;;) {
@@ -1244,6 +1276,12 @@ extern "C" amx_status amx_fd_time_integration(amx_fd_ctx* c, float* state, const
       break;
     }
     if (run->max_steps >= 0 && steps >= run->max_steps) break;
+    if (snp) {
+      amx_status s = snp->begin(t, state, st);
+      if (s != AMX_OK) return s;
+      sp.clamp_tp = snp->clamp();
+      if (sp.clamp_tp) sp.tp = snp->clamp_target();
+    }
     if (!run->fixed_timestep) AMX_HIPCHK(fd_cfl(c, state, st));
     hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(1), 0, st, c->ctl, sp);
     AMX_HIPCHK(hipGetLastError());
@@ -1256,7 +1294,12 @@ extern "C" amx_status amx_fd_time_integration(amx_fd_ctx* c, float* state, const
     fd_tresolve(c);
     t = c->hctl->t;
     ++steps;
+    if (snp) {
+      s = snp->end(t, state, st);
+      if (s != AMX_OK) return s;
+    }
   }
+  if (snp) snp->finish(steps);
   if (stats) {
     stats->steps = steps;
     stats->t = t;
@@ -1264,6 +1307,38 @@ extern "C" amx_status amx_fd_time_integration(amx_fd_ctx* c, float* state, const
     stats->max_fixed_point_iterations = 0;
   }
   return AMX_OK;
+}
+
+}  // namespace
+
+extern "C" amx_status amx_fd_time_integration(amx_fd_ctx* c, float* state, const amx_fv_run* run, amx_fv_stats* stats,
+                                              void* stream) {
+  if (!c || !state || !run) return amx_fail(AMX_ERR_INVALID, "null argument%s");
+  if (run->fixed_timestep && run->num_timesteps <= 0) return amx_fail(AMX_ERR_INVALID, "num_timesteps must be positive%s");
+  return fd_loop(c, state, run, nullptr, stats, (hipStream_t)stream);
+}
+
+extern "C" amx_status amx_fd_time_integration_snapshots(amx_fd_ctx* c, float* state, const amx_fv_run* run,
+                                                        const amx_snapshot_spec* spec, amx_snapshot_out* out,
+                                                        amx_fv_stats* stats, void* stream) {
+  if (!c || !state || !run || !spec) return amx_fail(AMX_ERR_INVALID, "null argument%s");
+  if (run->fixed_timestep && run->num_timesteps <= 0) return amx_fail(AMX_ERR_INVALID, "num_timesteps must be positive%s");
+  amx::snap::Driver d;
+  d.spec = spec;
+  d.out = out;
+  d.desc = fd_diag_desc(c, spec->box_size);
+  d.t_end = run->t_end;
+  d.work = nullptr;
+  amx_status s = amx::snap::validate(spec, out, d.desc);
+  if (s != AMX_OK) return s;
+  if (!spec->callback) {
+    if (!c->snapwork && hipMalloc(&c->snapwork, amx::snap::work_bytes()) != hipSuccess) {
+      c->snapwork = nullptr;
+      return amx_fail(AMX_ERR_NOMEM, "snapshot workspace allocation failed%s");
+    }
+    d.work = c->snapwork;
+  }
+  return fd_loop(c, state, run, &d, stats, (hipStream_t)stream);
 }
 
 extern "C" amx_status amx_fd_set_timing(amx_fd_ctx* c, int enable) {

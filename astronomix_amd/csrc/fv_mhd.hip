@@ -35,6 +35,7 @@
 
 #include "amx.h"
 #include "common.h"
+#include "snapshots.h"
 
 namespace amx {
 
@@ -962,6 +963,8 @@ struct StepParams {
   float C;
   float dx;
   int dim;
+  int clamp_tp = 0;  // clamp dt to tp - t (snapshot time points, time_integration.py:671-674)
+  float tp = 0.f;
 };
 
 // dt = C / sum_a(max_a / dx) (_timestep_estimator.py:128), or t_end/num_timesteps
@@ -1006,6 +1009,10 @@ __global__ void k_step_begin(Ctl* ctl, StepParams p) {
   } else {
     cfl_reduce(c, p.dim);
     dt = cfl_dt_from(c, p.C, p.dx, p.dim);
+  }
+  if (p.clamp_tp) {
+    const float rem = p.tp - c->t;
+    dt = rem < dt ? rem : dt;
   }
   if (p.exact_end) {
     const float rem = c->t_end - c->t;
@@ -1136,6 +1143,7 @@ struct amx_fv_ctx {
   int stepping;     // pool: a step is launched and not finished
   hipEvent_t ev_status;
   float* adj;       // reverse-mode workspace (allocated on first use)
+  void* snapwork;   // snapshot diagnostics partials (allocated on first use)
   int klast;        // pool: last fixed-point launch of the step in flight
   int use_graphs;   // pool: replay captured HIP graphs of the step halves
   std::map<std::pair<int, int>, hipGraphExec_t> g_launch;  // (span, klast)
@@ -1240,6 +1248,7 @@ void free_ctx(amx_fv_ctx* c) {
   (void)hipDeviceSynchronize();
   if (c->mem) (void)hipFree(c->mem);
   if (c->adj) (void)hipFree(c->adj);
+  if (c->snapwork) (void)hipFree(c->snapwork);
   if (c->ctl) (void)hipFree(c->ctl);
   if (c->dph) (void)hipFree(c->dph);
   if (c->hctl) (void)hipHostFree(c->hctl);
@@ -1729,12 +1738,31 @@ extern "C" amx_status amx_fv_evolve_state(amx_fv_ctx* c, float* state, const flo
   return AMX_OK;
 }
 
-extern "C" amx_status amx_fv_time_integration(amx_fv_ctx* c, float* state, const amx_fv_run* run,
-                                              amx_fv_stats* stats, void* stream) {
-  if (!c || !state || !run) return fail(AMX_ERR_INVALID, "null argument%s");
-  if (c->own_state) return fail(AMX_ERR_INVALID, "a pool context is stepped with amx_fv_pool_*%s");
-  if (run->fixed_timestep && run->num_timesteps <= 0) return fail(AMX_ERR_INVALID, "num_timesteps must be positive%s");
-  hipStream_t st = (hipStream_t)stream;
+namespace {
+
+// Layout of a single-simulation FV state for the snapshot diagnostics.
+amx_diag_desc fv_diag_desc(const amx_fv_ctx* c, double box_size) {
+  amx_diag_desc d;
+  d.num_vars = c->NS;
+  d.dim = c->cfg.dim;
+  d.nx = c->cfg.nx;
+  d.ny = c->cfg.ny;
+  d.nz = c->cfg.dim == 3 ? c->cfg.nz : 1;
+  d.rho_row = 0;
+  d.vel_row = 1;
+  d.p_row = c->cfg.dim + 1;
+  d.mag_row = c->cfg.mhd ? c->cfg.dim + 2 : 0;
+  d.div_mode = c->cfg.mhd ? AMX_DIV_CENTRAL : AMX_DIV_NONE;
+  d.gamma = c->cfg.gamma;
+  d.grid_spacing = c->cfg.grid_spacing;
+  d.box_size = box_size;
+  return d;
+}
+
+// The step loop of _time_integration (time_integration.py:629-801); with a
+// snapshot driver, rows are taken as in time_integration.py:539-610, 736-745.
+amx_status fv_loop(amx_fv_ctx* c, float* state, const amx_fv_run* run, snap::Driver* snp, amx_fv_stats* stats,
+                   hipStream_t st) {
   hipLaunchKernelGGL(k_slot_reset, dim3(1), dim3(1), 0, st, c->ctl, 0, run->t_end, 1);
   HIPCHK(hipGetLastError());
   if (!run->fixed_timestep) HIPCHK(launch_cfl(c, state, 0, 1, st));
@@ -1745,6 +1773,8 @@ extern "C" amx_status amx_fv_time_integration(amx_fv_ctx* c, float* state, const
   sp.C = c->cfg.C_cfl;
   sp.dx = c->dx;
   sp.dim = c->cfg.dim;
+  // the end-time clamp is off with snapshot time points (time_integration.py:677)
+  if (snp && snp->spec->use_timepoints) sp.exact_end = 0;
   long steps = 0, iters = 0;
   int kmax = 0;
   float t = 0.f;
@@ -1755,6 +1785,12 @@ extern "C" amx_status amx_fv_time_integration(amx_fv_ctx* c, float* state, const
       break;
     }
     if (run->max_steps >= 0 && steps >= run->max_steps) break;
+    if (snp) {
+      amx_status s = snp->begin(t, state, st);
+      if (s != AMX_OK) return s;
+      sp.clamp_tp = snp->clamp();
+      if (sp.clamp_tp) sp.tp = snp->clamp_target();
+    }
     hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(1), 0, st, c->ctl, sp);
     HIPCHK(hipGetLastError());
     int K = 0;
@@ -1775,10 +1811,15 @@ extern "C" amx_status amx_fv_time_integration(amx_fv_ctx* c, float* state, const
     iters += K;
     if (K > kmax) kmax = K;
     ++steps;
+    if (snp) {
+      s = snp->end(t, state, st);
+      if (s != AMX_OK) return s;
+    }
   }
   HIPCHK(hipMemcpyAsync(c->hctl, c->ctl, offsetof(Ctl, cflp), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   time_resolve(c, 1 << 30);
+  if (snp) snp->finish(steps);
   if (stats) {
     stats->steps = steps;
     stats->t = c->hctl[0].t;
@@ -1786,6 +1827,40 @@ extern "C" amx_status amx_fv_time_integration(amx_fv_ctx* c, float* state, const
     stats->max_fixed_point_iterations = kmax;
   }
   return AMX_OK;
+}
+
+}  // namespace
+
+extern "C" amx_status amx_fv_time_integration(amx_fv_ctx* c, float* state, const amx_fv_run* run,
+                                              amx_fv_stats* stats, void* stream) {
+  if (!c || !state || !run) return fail(AMX_ERR_INVALID, "null argument%s");
+  if (c->own_state) return fail(AMX_ERR_INVALID, "a pool context is stepped with amx_fv_pool_*%s");
+  if (run->fixed_timestep && run->num_timesteps <= 0) return fail(AMX_ERR_INVALID, "num_timesteps must be positive%s");
+  return fv_loop(c, state, run, nullptr, stats, (hipStream_t)stream);
+}
+
+extern "C" amx_status amx_fv_time_integration_snapshots(amx_fv_ctx* c, float* state, const amx_fv_run* run,
+                                                        const amx_snapshot_spec* spec, amx_snapshot_out* out,
+                                                        amx_fv_stats* stats, void* stream) {
+  if (!c || !state || !run || !spec) return fail(AMX_ERR_INVALID, "null argument%s");
+  if (c->own_state) return fail(AMX_ERR_INVALID, "a pool context is stepped with amx_fv_pool_*%s");
+  if (run->fixed_timestep && run->num_timesteps <= 0) return fail(AMX_ERR_INVALID, "num_timesteps must be positive%s");
+  snap::Driver d;
+  d.spec = spec;
+  d.out = out;
+  d.desc = fv_diag_desc(c, spec->box_size);
+  d.t_end = run->t_end;
+  d.work = nullptr;
+  amx_status s = snap::validate(spec, out, d.desc);
+  if (s != AMX_OK) return s;
+  if (!spec->callback) {
+    if (!c->snapwork && hipMalloc(&c->snapwork, snap::work_bytes()) != hipSuccess) {
+      c->snapwork = nullptr;
+      return fail(AMX_ERR_NOMEM, "snapshot workspace allocation failed%s");
+    }
+    d.work = c->snapwork;
+  }
+  return fv_loop(c, state, run, &d, stats, (hipStream_t)stream);
 }
 
 // ============================================================================

@@ -243,6 +243,91 @@ amx_status amx_fd_interface_fields(const float* B, float* b, int nx, int ny, int
 amx_status amx_fd_set_timing(amx_fd_ctx* ctx, int enable);
 amx_status amx_fd_get_timing(amx_fd_ctx* ctx, double* weno_ms, long* weno_launches);
 
+/* ---------------------------------------------------------------------------
+ * Snapshots and diagnostics: the output contract of time_integration with
+ * config.return_snapshots (SnapshotData, data_classes/simulation_snapshot_data.py:5-49;
+ * time_integration.py:297-369 set-up, :404-537 one snapshot row, :539-610
+ * schedule and callbacks, :671-678 dt clamps, :736-745 end-time row).
+ * ------------------------------------------------------------------------- */
+
+/* Magnetic-divergence stencils of the max |div B| diagnostic. */
+#define AMX_DIV_NONE 0     /* no magnetic field */
+#define AMX_DIV_CENTRAL 1  /* cell-centred B, (B[i+1] - B[i-1]) / (2 dx) per axis
+                              (divergence3D, _finite_volume/_magnetic_update/_vector_maths.py:33-51) */
+#define AMX_DIV_FACES 2    /* face B, sixth-order interface difference
+                              (_interface_field_divergence, _finite_difference/_maths/_differencing.py:9-52) */
+
+/* Where the diagnostics find the fields of a state (num_vars, nx, ny[, nz]). */
+typedef struct amx_diag_desc {
+  int num_vars;                   /* rows of the state */
+  int dim;                        /* 2 or 3 */
+  int nx, ny, nz;                 /* nz = 1 in 2D */
+  int rho_row;                    /* registered_variables.density_index */
+  int vel_row;                    /* first of dim velocity rows */
+  int p_row;                      /* registered_variables.pressure_index */
+  int mag_row;                    /* first of 3 magnetic rows: cell-centred (CENTRAL) or face (FACES) */
+  int div_mode;                   /* AMX_DIV_* (CENTRAL needs dim == 3, as divergence3D does) */
+  float gamma;                    /* params.gamma */
+  double grid_spacing;            /* config.grid_spacing */
+  double box_size;                /* config.box_size */
+} amx_diag_desc;
+
+/* Snapshot arrays: device pointers to num_snapshots floats each (states:
+ * num_snapshots whole states); NULL = the quantity is not recorded (its
+ * SnapshotSettings flag is off). The last two fields are outputs. */
+typedef struct amx_snapshot_out {
+  float* time_points;             /* SnapshotData.time_points */
+  float* states;                  /* SnapshotData.states */
+  float* total_mass;              /* sum(rho) * box_size^dim (total_quantities.py:155-176) */
+  float* total_energy;            /* sum(p/(gamma-1) + rho u^2/2) dx^dim (total_quantities.py:107-150) */
+  float* internal_energy;         /* sum(p/(gamma-1)) dx^dim (total_quantities.py:29-43) */
+  float* kinetic_energy;          /* sum(rho u^2/2) dx^dim (total_quantities.py:74-85) */
+  float* radial_momentum;         /* sum(rho u.r_hat) dx^dim (total_quantities.py:46-71) */
+  float* magnetic_divergence;     /* max |div B| (time_integration.py:507-522) */
+  int current_checkpoint;         /* out: snapshots taken (SnapshotData.current_checkpoint) */
+  long num_iterations;            /* out: steps taken (SnapshotData.num_iterations) */
+} amx_snapshot_out;
+
+/* Callback mode (config.activate_snapshot_callback, time_integration.py:571-610):
+ * called on the host, with the stream drained, with the simulation time, the
+ * checkpoint index and the device state (read only). Non-zero stops the run
+ * with AMX_ERR_INVALID. */
+typedef int (*amx_snapshot_callback)(void* user, float t, int checkpoint, const float* state);
+
+typedef struct amx_snapshot_spec {
+  int num_snapshots;              /* config.num_snapshots */
+  int use_timepoints;             /* config.use_specific_snapshot_timepoints */
+  const float* timepoints;        /* host array, params.snapshot_timepoints */
+  int num_timepoints;             /* its length (>= 1 when use_timepoints) */
+  double box_size;                /* config.box_size */
+  amx_snapshot_callback callback; /* NULL: record into amx_snapshot_out; else callback mode */
+  void* user;                     /* passed to callback */
+} amx_snapshot_spec;
+
+/* Bytes of the device workspace amx_snapshot_record needs (block partials). */
+size_t amx_snapshot_workspace_bytes(void);
+
+/* One snapshot row (update_snapshot_data, time_integration.py:404-537):
+ * out->time_points[index] = t, out->states[index] = state, and each non-NULL
+ * diagnostic at [index], for the state described by d. Sums accumulate in
+ * float64 in a fixed order (deterministic); the max is exact. work: device
+ * buffer of amx_snapshot_workspace_bytes(). Stream-ordered. */
+amx_status amx_snapshot_record(const amx_diag_desc* d, const float* state, float t, const amx_snapshot_out* out,
+                               int index, void* work, void* stream);
+
+/* The step loops of amx_fv_time_integration / amx_fd_time_integration with
+ * the snapshot schedule of time_integration.py:539-610 (a row when
+ * t >= k t_end / num_snapshots at the start of a step, or, with timepoints,
+ * when |t - timepoints[k]| < 1e-12 at the start of a step or t reaches t_end;
+ * dt is then clamped to timepoints[k] - t and exact_end_time is off). Rows
+ * past num_snapshots are dropped, as JAX drops out-of-bounds updates. */
+amx_status amx_fv_time_integration_snapshots(amx_fv_ctx* ctx, float* state, const amx_fv_run* run,
+                                             const amx_snapshot_spec* spec, amx_snapshot_out* out,
+                                             amx_fv_stats* stats, void* stream);
+amx_status amx_fd_time_integration_snapshots(amx_fd_ctx* ctx, float* state, const amx_fv_run* run,
+                                             const amx_snapshot_spec* spec, amx_snapshot_out* out,
+                                             amx_fv_stats* stats, void* stream);
+
 /* Plain device copy dst[i] = src[i], i < n (16-byte vector loads/stores).
  * No counterpart in the reference: the HBM ceiling (STREAM copy) the
  * benchmark reports next to the kernels, and the known-size kernel against

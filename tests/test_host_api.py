@@ -89,12 +89,15 @@ def test_implemented_path_gate():
     assert "2D and 3D" in unsupported_reason(ok._replace(dimensionality=1))
 
 
-def test_time_integration_rejects_snapshots_before_touching_the_gpu():
+def test_time_integration_rejects_unsupported_options_before_touching_the_gpu():
+    import torch
     state, cfg, params = W.workload("orszag_tang_2d", 16)
+    rv = amx.get_registered_variables(cfg)
     with pytest.raises(NotImplementedError):
-        amx.time_integration(state, cfg._replace(return_snapshots=True), params, amx.get_registered_variables(cfg))
-    with pytest.raises(NotImplementedError):
-        amx.time_integration(state, cfg._replace(split=sc.SPLIT), params, amx.get_registered_variables(cfg))
+        amx.time_integration(state, cfg._replace(split=sc.SPLIT), params, rv)
+    grad_state = torch.from_numpy(state).requires_grad_(True)
+    with pytest.raises(NotImplementedError, match="snapshots"):
+        amx.time_integration(grad_state, cfg._replace(return_snapshots=True), params, rv)
 
 
 def header_symbols():
@@ -115,13 +118,35 @@ def test_library_exports_every_declared_symbol():
     assert set(syms) <= set(_native.SIGNATURES), set(syms) - set(_native.SIGNATURES)
 
 
-def test_native_struct_layout_matches_header():
-    # field order/types mirrored from include/amx.h; sizes pin the padding
-    assert ctypes.sizeof(_native.AmxFvConfig) == 56
-    assert ctypes.sizeof(_native.AmxFvRun) == 24
-    assert ctypes.sizeof(_native.AmxFvStats) == 32
-    assert ctypes.sizeof(_native.AmxFdConfig) == 48
-    assert ctypes.sizeof(_native.AmxFvSlotStatus) == 40
+STRUCTS = {  # C struct in include/amx.h -> ctypes mirror in _native
+    "amx_fv_config": _native.AmxFvConfig, "amx_fv_run": _native.AmxFvRun, "amx_fv_stats": _native.AmxFvStats,
+    "amx_fv_timing": _native.AmxFvTiming, "amx_fv_slot_status": _native.AmxFvSlotStatus,
+    "amx_fd_config": _native.AmxFdConfig, "amx_diag_desc": _native.AmxDiagDesc,
+    "amx_snapshot_out": _native.AmxSnapshotOut, "amx_snapshot_spec": _native.AmxSnapshotSpec,
+}
+
+
+def test_native_struct_layout_matches_header(tmp_path):
+    """Size and every field offset of each ctypes mirror equal the C
+    compiler's layout of the header's struct."""
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "amx.h"', "int main(void) {"]
+    for cname, cls in STRUCTS.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in cls._fields_:
+            lines.append(f'  printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines():
+        cname, field, value = line.split()
+        got[(cname, field)] = int(value)
+    for cname, cls in STRUCTS.items():
+        assert got[(cname, "size")] == ctypes.sizeof(cls), cname
+        for f, _ in cls._fields_:
+            assert got[(cname, f)] == getattr(cls, f).offset, (cname, f)
 
 
 def test_no_gpu_means_loud_failure(monkeypatch):
