@@ -7,8 +7,8 @@
 //               the CFL maxima of its output and the copy of the final B.
 //               Follows _evolve_gas_state_unsplit_inner / _evolve_gas_state_unsplit
 //               (astronomix/_finite_volume/_state_evolution/evolve_state.py:338-469).
-//   * k_mag     one fixed-point iterate of the implicit-midpoint magnetic
-//               sub-step with its convergence residual
+//   * k_mag     two consecutive fixed-point iterates of the implicit-midpoint
+//               magnetic sub-step with their convergence residuals
 //               (astronomix/_finite_volume/_magnetic_update/_magnetic_field_update.py:82-165).
 //   * k_cfl     the unsplit CFL maxima (_timestep_estimator.py:118-128).
 //
