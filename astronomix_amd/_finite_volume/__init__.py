@@ -11,15 +11,21 @@ from __future__ import annotations
 from astronomix_amd.option_classes.simulation_config import (
     ALL_PERIODIC,
     CARTESIAN,
+    DOUBLE_MINMOD,
     DOUBLE_PRECISION,
     FINITE_VOLUME,
     HLL,
     HLLC,
+    HLLC_LM,
     IMPLICIT_EULER,
     IMPLICIT_MIDPOINT,
+    LAX_FRIEDRICHS,
     MINMOD,
+    OSHER,
     RK2_SSP,
     SPLIT,
+    SUPERBEE,
+    VAN_ALBADA,
     BoundarySettings,
 )
 from astronomix_amd import _native
@@ -29,8 +35,10 @@ def unsupported_reason(config, registered_variables=None):
     """None when ``config`` lies on the implemented path, else why not.
 
     The implemented path is the unsplit SSP-RK2 finite-volume scheme with the
-    minmod limiter, HLL/HLLC fluxes, periodic boundaries on every axis, 2D or
-    3D, hydro or MHD (Pang-Wu magnetic sub-step), float32, no physics modules.
+    minmod, Osher, double-minmod, superbee or van Albada limiter, HLL, HLLC,
+    HLLC-LM or Lax-Friedrichs fluxes, periodic boundaries on every axis, 2D
+    or 3D, hydro or MHD (Pang-Wu magnetic sub-step), float32, no physics
+    modules.
     """
     c = config
     if c.solver_mode != FINITE_VOLUME:
@@ -45,10 +53,10 @@ def unsupported_reason(config, registered_variables=None):
         return "only the UNSPLIT scheme is implemented"
     if c.time_integrator != RK2_SSP:
         return "only the RK2_SSP integrator is implemented for the unsplit scheme"
-    if c.limiter != MINMOD:
-        return "only the MINMOD limiter is implemented"
-    if c.riemann_solver not in (HLL, HLLC):
-        return "only the HLL and HLLC Riemann solvers are implemented"
+    if c.limiter not in (MINMOD, OSHER, DOUBLE_MINMOD, SUPERBEE, VAN_ALBADA):
+        return "the VAN_ALBADA_PP limiter is not implemented"
+    if c.riemann_solver not in (HLL, HLLC, HLLC_LM, LAX_FRIEDRICHS):
+        return "the HYBRID_HLLC and AM_HLLC Riemann solvers are not implemented"
     if c.fv_magnetic_integrator not in (IMPLICIT_MIDPOINT, IMPLICIT_EULER):
         return "unknown fv_magnetic_integrator"
     if c.first_order_fallback:
@@ -92,7 +100,7 @@ def native_config(state_shape, config, gamma, C_cfl, grid_spacing=None) -> _nati
     h = config.grid_spacing if grid_spacing is None else grid_spacing
     return _native.AmxFvConfig(
         dim=dim, nx=nx, ny=ny, nz=nz, mhd=int(bool(config.mhd)),
-        riemann=_native.AMX_HLLC if config.riemann_solver == HLLC else _native.AMX_HLL,
+        riemann=int(config.riemann_solver), limiter=int(config.limiter),
         magnetic_integrator=int(config.fv_magnetic_integrator), max_fixed_point_iterations=1000,
         gamma=float(gamma), C_cfl=float(C_cfl), fixed_point_tol=1e-5, grid_spacing=float(h))
 

@@ -17,6 +17,9 @@ LIB_PATH = os.environ.get("AMX_LIB") or os.path.join(_HERE, "libamx.so")
 AMX_OK = 0
 AMX_HLL = 0
 AMX_HLLC = 1
+AMX_HLLC_LM = 2
+AMX_LAX_FRIEDRICHS = 3
+AMX_MINMOD = 0
 
 
 class AmxFvConfig(ctypes.Structure):
@@ -32,6 +35,7 @@ class AmxFvConfig(ctypes.Structure):
         ("gamma", ctypes.c_float),
         ("C_cfl", ctypes.c_float),
         ("fixed_point_tol", ctypes.c_float),
+        ("limiter", ctypes.c_int),
         ("grid_spacing", ctypes.c_double),
     ]
 

@@ -23,6 +23,7 @@ import numpy as np
 
 from astronomix_amd import _native
 from astronomix_amd._finite_volume import native_config, require_supported
+from astronomix_amd.option_classes.simulation_config import HLL, HLLC, MINMOD
 
 
 def _function():
@@ -104,6 +105,8 @@ def rollout(state, config, params, registered_variables=None, checkpoint_every: 
     require_supported(config, registered_variables)
     if config.mhd:
         raise NotImplementedError("reverse mode covers the hydro step only (the magnetic fixed point has none)")
+    if config.riemann_solver not in (HLL, HLLC) or config.limiter != MINMOD:
+        raise NotImplementedError("reverse mode covers the HLL / HLLC fluxes with the minmod limiter")
     if not isinstance(state, torch.Tensor) or state.device.type != "cuda" or state.dtype != torch.float32:
         raise TypeError("state must be a float32 tensor on the GPU")
     _native.require_gpu()

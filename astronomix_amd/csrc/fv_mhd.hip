@@ -66,6 +66,7 @@ struct Ctl {
   unsigned cfl[4];              // reduced CFL maxima per axis
   unsigned cflp[3][NSLOT];      // CFL partial maxima
   unsigned resp[NRES][NSLOT];   // fixed-point residual partials, by launch index % NRES
+  unsigned lfp[3][NSLOT];       // Lax-Friedrichs: partial maxima of |u| + c of the stage input
 };
 
 struct Phys {
@@ -86,6 +87,42 @@ __device__ __forceinline__ float minmod(float a, float b) { return __builtin_amd
 // limited gradient times dx/2 (limited_gradients.py:50-60, reconstruction.py:409-412)
 __device__ __forceinline__ float half_slope(float wm, float w0, float wp) {
   return 0.5f * minmod(w0 - wm, wp - w0);
+}
+
+// Limiter ids (simulation_config.py:29-34 values).
+enum { LIM_MINMOD = 0, LIM_OSHER = 1, LIM_DOUBLE_MINMOD = 2, LIM_SUPERBEE = 3, LIM_VAN_ALBADA = 4 };
+
+// The other limiters of _calculate_limited_gradients (limited_gradients.py:49-88,
+// limiters.py:18-64) on the backward / forward gradients b, f, chosen by a
+// block-uniform id; the result is (g dx) / 2 as the reconstruction applies it.
+__device__ __forceinline__ float half_slope_x(float wm, float w0, float wp, int lim, float dx) {
+  const float b = (w0 - wm) / dx, f = (wp - w0) / dx;
+  float g;
+  if (lim == LIM_SUPERBEE) {
+    // where(a b > 0, minmod(maxmod(a, b), minmod(2a, 2b)), 0): for a, b of one
+    // sign s that is s min(max(|a|,|b|), 2 min(|a|,|b|))
+    const float aa = fabsf(b), ab = fabsf(f);
+    g = b * f > 0.f ? copysignf(fminf(fmaxf(aa, ab), 2.f * fminf(aa, ab)), b) : 0.f;
+  } else if (lim == LIM_DOUBLE_MINMOD) {
+    // where(a b > 0, minmod3((a + b) / 2, 2a, 2b), 0)
+    const float m = (b + f) / 2.f;
+    g = b * f > 0.f ? copysignf(fminf(fabsf(m), fminf(fabsf(2.f * b), fabsf(2.f * f))), m) : 0.f;
+  } else if (lim == LIM_VAN_ALBADA) {
+    // ((f^2 + eps) b + (b^2 + eps) f) / (f^2 + b^2 + 2 eps), eps = 3 dx
+    const float eps = 3.f * dx, eps2 = 6.f * dx;
+    g = ((f * f + eps) * b + (b * b + eps) * f) / ((f * f + b * b) + eps2);
+  } else {  // LIM_OSHER: b max(0, min(1.3, f / (b + 1e-11))) where |b| > 1e-11
+    const float r = fabsf(b) > 1e-11f ? f / (b + 1e-11f) : 0.f;
+    g = fmaxf(0.f, fminf(1.3f, r)) * b;
+  }
+  return (g * dx) / 2.f;
+}
+
+// Limited half slope: minmod in its exact difference form, or another limiter.
+template <bool LIMX>
+__device__ __forceinline__ float slope(float wm, float w0, float wp, int lim, float dx) {
+  if (LIMX) return half_slope_x(wm, w0, wp, lim, dx);
+  return half_slope(wm, w0, wp);
 }
 
 template <int DIM>
@@ -149,8 +186,10 @@ __device__ __forceinline__ void flux_hll(const float* wl, const float* wr, float
   f[P] = (al * (ul * (pl + El)) + ar * (ur * (pr + Er))) + as * (Er - El);
 }
 
-// HLLC interface flux, central form (_finite_volume/_riemann_solver/hll.py:113-261).
-template <int DIM, int AX>
+// HLLC interface flux, central form (_finite_volume/_riemann_solver/hll.py:113-261);
+// LM: the low-Mach variant HLLC-LM, whose dissipation takes S_L, S_R scaled
+// by phi = sin(min(1, Ma / 0.1) pi / 2) (hll.py:206-221).
+template <int DIM, int AX, bool LM>
 __device__ __forceinline__ void flux_hllc(const float* wl, const float* wr, float* f, const Phys& ph) {
   constexpr int P = DIM + 1, NV = DIM + 2;
   const float rl = wl[0], ul = wl[AX], pl = wl[P];
@@ -189,6 +228,13 @@ __device__ __forceinline__ void flux_hllc(const float* wl, const float* wr, floa
   const float fl = (SL - ul) * frcp(SL - s_star);
   const float fr = (SR - ur) * frcp(SR - s_star);
   const float as = fabsf(s_star);
+  float SLd = SL, SRd = SR;
+  if (LM) {
+    const float ma = fmaxf(fabsf(ul / cl), fabsf(ur / cr));
+    const float phi = sinf((fminf(1.f, ma / 0.1f) * 3.14159265358979f) / 2.f);
+    SLd = SL * phi;
+    SRd = SR * phi;
+  }
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
     float usl = UL[v], usr = UR[v];
@@ -202,16 +248,46 @@ __device__ __forceinline__ void flux_hllc(const float* wl, const float* wr, floa
     }
     usl *= fl;
     usr *= fr;
-    const float fs = 0.5f * (FL[v] + FR[v]) + 0.5f * ((SL * (usl - UL[v]) + as * (usl - usr)) + SR * (usr - UR[v]));
+    const float fs = 0.5f * (FL[v] + FR[v]) + 0.5f * ((SLd * (usl - UL[v]) + as * (usl - usr)) + SRd * (usr - UR[v]));
     float out = SL >= 0.f ? FL[v] : fs;
     f[v] = SR <= 0.f ? FR[v] : out;
   }
 }
 
+// Lax-Friedrichs flux (_finite_volume/_riemann_solver/_lax_friedrichs.py:21-78):
+// 0.5 (F_L + F_R) - 0.5 alpha (U_R - U_L) with alpha = max over the stage
+// input of |u| + c along the flux direction (a grid-wide maximum).
+template <int DIM, int AX>
+__device__ __forceinline__ void flux_lf(const float* wl, const float* wr, float* f, const Phys& ph, float alpha) {
+  constexpr int P = DIM + 1;
+  const float rl = wl[0], ul = wl[AX], pl = wl[P];
+  const float rr = wr[0], ur = wr[AX], pr = wr[P];
+  const float El = pl * ph.inv_gm1 + (0.5f * rl) * vsq<DIM>(wl);
+  const float Er = pr * ph.inv_gm1 + (0.5f * rr) * vsq<DIM>(wr);
+  const float ha = 0.5f * alpha;
+  f[0] = 0.5f * (ul * rl + ur * rr) - ha * (rr - rl);
+#pragma unroll
+  for (int i = 1; i <= DIM; ++i) {
+    const float ml = rl * wl[i], mr = rr * wr[i];
+    float fl = ul * ml, fr = ur * mr;
+    if (i == AX) {
+      fl += pl;
+      fr += pr;
+    }
+    f[i] = 0.5f * (fl + fr) - ha * (mr - ml);
+  }
+  f[P] = 0.5f * (ul * (pl + El) + ur * (pr + Er)) - ha * (Er - El);
+}
+
+// Riemann solver ids (simulation_config.py:41-46 values).
 template <int DIM, int AX, int RS>
-__device__ __forceinline__ void riemann(const float* wl, const float* wr, float* f, const Phys& ph) {
+__device__ __forceinline__ void riemann(const float* wl, const float* wr, float* f, const Phys& ph, float alpha) {
   if (RS == AMX_HLLC)
-    flux_hllc<DIM, AX>(wl, wr, f, ph);
+    flux_hllc<DIM, AX, false>(wl, wr, f, ph);
+  else if (RS == AMX_HLLC_LM)
+    flux_hllc<DIM, AX, true>(wl, wr, f, ph);
+  else if (RS == AMX_LAX_FRIEDRICHS)
+    flux_lf<DIM, AX>(wl, wr, f, ph, alpha);
   else
     flux_hll<DIM, AX>(wl, wr, f, ph);
 }
@@ -301,6 +377,8 @@ struct StageArgs {
   int nx, na, nb;        // 3D: (nx, ny, nz); 2D: (1, nx, ny)
   int ntb, nta, ncx, lx;
   float dt_scale;        // 0.5: MHD half step dt/2; 1: hydro step
+  int limiter;           // LIM_* (the kernels instantiated with LIMX read it)
+  float dxf;             // float(grid_spacing): the limiters' gradient scale
 };
 
 // One unsplit forward-Euler stage (evolve_state.py:338-414), optionally with
@@ -312,7 +390,7 @@ struct StageArgs {
 // (TA+1)*64 A-interfaces and TA*65 B-interfaces of a plane take exactly two
 // rounds of the block. In 3D the block marches along x over lx planes with the
 // x-stencil of its own column in registers.
-template <int DIM, int TA, int SRC, bool HEUN, int W0M, int RS>
+template <int DIM, int TA, int SRC, bool HEUN, int W0M, int RS, bool LIMX>
 __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(AMX_STAGE_WAVES))) void k_stage(const StageArgs a) {
   constexpr int NV = DIM + 2, P = DIM + 1, NT = (TA + 1) * TB, WW = TB + 4;
   constexpr int AXA = DIM == 3 ? 2 : 1;  // velocity row of the plane axis A
@@ -350,6 +428,14 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
   for (int v = 0; v < NV; ++v) outp[v] = a.out[v] + so;
   const Phys ph = a.phs[slot];
   const int tid = threadIdx.x, ta = tid / TB, tb = tid % TB;
+  const int lim = a.limiter;
+  const float ldx = a.dxf;
+  // Lax-Friedrichs: alpha per velocity row from the stage input's maxima
+  float alf[3] = {0.f, 0.f, 0.f};
+  if (RS == AMX_LAX_FRIEDRICHS) {
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) alf[k] = __uint_as_float(wave_max_u(C->lfp[k][tid & 63]));
+  }
   const bool owner = ta < TA;
   const int bid = swizzle_block(blockIdx.x, gridDim.x);
   const int itb = bid % a.ntb;
@@ -398,11 +484,11 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
       load_w<DIM, SRC>(src, (unsigned)wrapi(x0 + 1, a.nx) * sx + col, q1);
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
-        d0[v] = half_slope(qm1[v], q0[v], q1[v]);
-        wl[v] = qm1[v] + half_slope(qm2[v], qm1[v], q0[v]);
+        d0[v] = slope<LIMX>(qm1[v], q0[v], q1[v], lim, ldx);
+        wl[v] = qm1[v] + slope<LIMX>(qm2[v], qm1[v], q0[v], lim, ldx);
         wr[v] = q0[v] - d0[v];
       }
-      riemann<DIM, 1, RS>(wl, wr, Fp, ph);
+      riemann<DIM, 1, RS>(wl, wr, Fp, ph, alf[0]);
     } else {
       load_w<DIM, SRC>(src, col, q0);
     }
@@ -437,12 +523,12 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
         float d1[NV], wl[NV], wr[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-          d1[v] = half_slope(q0[v], q1[v], q2[v]);
+          d1[v] = slope<LIMX>(q0[v], q1[v], q2[v], lim, ldx);
           wl[v] = q0[v] + d0[v];
           wr[v] = q1[v] - d1[v];
           d0[v] = d1[v];
         }
-        riemann<DIM, 1, RS>(wl, wr, Fn, ph);
+        riemann<DIM, 1, RS>(wl, wr, Fn, ph, alf[0]);
       }
 #pragma unroll
       for (int v = 0; v < NV; ++v) sW[v][(ta + 2) * WW + tb + 2] = q0[v];
@@ -462,10 +548,10 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
         const float w0 = sW[v][o - 2 * WW], w1 = sW[v][o - WW], w2 = sW[v][o], w3 = sW[v][o + WW];
-        wl[v] = w1 + half_slope(w0, w1, w2);
-        wr[v] = w2 - half_slope(w1, w2, w3);
+        wl[v] = w1 + slope<LIMX>(w0, w1, w2, lim, ldx);
+        wr[v] = w2 - slope<LIMX>(w1, w2, w3, lim, ldx);
       }
-      riemann<DIM, AXA, RS>(wl, wr, f, ph);
+      riemann<DIM, AXA, RS>(wl, wr, f, ph, alf[AXA - 1]);
 #pragma unroll
       for (int v = 0; v < NV; ++v) sFA[v][tid] = f[v];
     }
@@ -476,10 +562,10 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
         const float w0 = sW[v][o - 2], w1 = sW[v][o - 1], w2 = sW[v][o], w3 = sW[v][o + 1];
-        wl[v] = w1 + half_slope(w0, w1, w2);
-        wr[v] = w2 - half_slope(w1, w2, w3);
+        wl[v] = w1 + slope<LIMX>(w0, w1, w2, lim, ldx);
+        wr[v] = w2 - slope<LIMX>(w1, w2, w3, lim, ldx);
       }
-      riemann<DIM, AXB, RS>(wl, wr, f, ph);
+      riemann<DIM, AXB, RS>(wl, wr, f, ph, alf[AXB - 1]);
 #pragma unroll
       for (int v = 0; v < NV; ++v) sFB[v][tid] = f[v];
     }
@@ -541,6 +627,55 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
       const unsigned m = block_max_u<TA + 1>(canon_bits(mx[k]), sred[k]);
       if (tid == 0 && m) atomicMax(&C->cflp[k][blockIdx.x & (NSLOT - 1)], m);
     }
+  }
+}
+
+// Lax-Friedrichs alpha of a stage input (_lax_friedrichs.py:61-66): per
+// velocity row k, the max over all cells of |u_k| + sqrt(gamma p / rho),
+// reduced into the slot's lfp[k] partials (block id mod 64) and read by
+// every block of the following stage kernel. k_lf_clear zeroes them first.
+constexpr int LF_BLOCKS = 512;
+
+__global__ void k_lf_clear(Ctl* ctl, int slot0) {
+  Ctl* const c = ctl + slot0 + blockIdx.x;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) c->lfp[k][threadIdx.x] = 0u;
+}
+
+template <int DIM, int SRC>
+__global__ __launch_bounds__(256) void k_lf_alpha(const StageArgs a) {
+  constexpr int P = DIM + 1;
+  __shared__ unsigned sred[4];
+  const int slot = a.slot0 + blockIdx.y;
+  Ctl* const C = a.ctl + slot;
+  if (!C->run) return;  // block-uniform
+  const long so = slot * a.sstride;
+  const int par = C->iters % 3;
+  const float* src[8];
+#pragma unroll
+  for (int v = 0; v < 8; ++v) src[v] = a.src[v] ? a.src[v] + so : nullptr;
+  if (SRC == SRC_MAGFIX) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) src[5 + k] = a.selv[par][k] + so;
+  }
+  const Phys ph = a.phs[slot];
+  const unsigned n = (unsigned)a.nx * (unsigned)a.na * (unsigned)a.nb;
+  float mx[3] = {0.f, 0.f, 0.f};
+  for (unsigned c = blockIdx.x * 256u + threadIdx.x; c < n; c += gridDim.x * 256u) {
+    float w[DIM + 2];
+    load_w<DIM, SRC>(src, c, w);
+    const float cs = fsqrt(ph.gamma * w[P] * frcp(w[0]));
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) {
+      const float m = fabsf(w[1 + k]) + cs;
+      mx[k] = (m != m || m > mx[k]) ? m : mx[k];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) {
+    const unsigned b = block_max_u<4>(canon_bits(mx[k]), sred);
+    if (threadIdx.x == 0 && b) atomicMax(&C->lfp[k][blockIdx.x & (NSLOT - 1)], b);
+    __syncthreads();  // sred is reused by the next axis
   }
 }
 
@@ -1217,7 +1352,10 @@ amx_status validate(const amx_fv_config* c) {
   if (c->dim != 2 && c->dim != 3) return fail(AMX_ERR_UNSUPPORTED, "dimensionality must be 2 or 3%s");
   if (c->nx < 2 || c->ny < 2 || (c->dim == 3 && c->nz < 2)) return fail(AMX_ERR_INVALID, "grid too small%s");
   if (c->dim == 2 && c->nz != 1) return fail(AMX_ERR_INVALID, "nz must be 1 in 2D%s");
-  if (c->riemann != AMX_HLL && c->riemann != AMX_HLLC) return fail(AMX_ERR_UNSUPPORTED, "riemann solver must be HLL or HLLC%s");
+  if (c->riemann < AMX_HLL || c->riemann > AMX_LAX_FRIEDRICHS)
+    return fail(AMX_ERR_UNSUPPORTED, "riemann solver must be HLL, HLLC, HLLC_LM or LAX_FRIEDRICHS%s");
+  if (c->limiter < AMX_MINMOD || c->limiter > AMX_VAN_ALBADA)
+    return fail(AMX_ERR_UNSUPPORTED, "limiter must be MINMOD, OSHER, DOUBLE_MINMOD, SUPERBEE or VAN_ALBADA%s");
   if (c->magnetic_integrator != AMX_IMPLICIT_MIDPOINT && c->magnetic_integrator != AMX_IMPLICIT_EULER)
     return fail(AMX_ERR_UNSUPPORTED, "unknown magnetic integrator%s");
   if (c->max_fixed_point_iterations < 0 || c->max_fixed_point_iterations > 100000)
@@ -1367,13 +1505,35 @@ Bufs bufs(const amx_fv_ctx* c, float* state) {
   return b;
 }
 
+template <int DIM, int SRC, bool HEUN, int W0M, bool LIMX>
+void launch_stage_rs(const amx_fv_ctx* c, const StageArgs& a, dim3 grid, dim3 block, hipStream_t st) {
+  switch (c->cfg.riemann) {
+    case AMX_HLLC:
+      hipLaunchKernelGGL((k_stage<DIM, TA_DEF, SRC, HEUN, W0M, AMX_HLLC, LIMX>), grid, block, 0, st, a);
+      break;
+    case AMX_HLLC_LM:
+      hipLaunchKernelGGL((k_stage<DIM, TA_DEF, SRC, HEUN, W0M, AMX_HLLC_LM, LIMX>), grid, block, 0, st, a);
+      break;
+    case AMX_LAX_FRIEDRICHS:
+      hipLaunchKernelGGL((k_stage<DIM, TA_DEF, SRC, HEUN, W0M, AMX_LAX_FRIEDRICHS, LIMX>), grid, block, 0, st, a);
+      break;
+    default:
+      hipLaunchKernelGGL((k_stage<DIM, TA_DEF, SRC, HEUN, W0M, AMX_HLL, LIMX>), grid, block, 0, st, a);
+  }
+}
+
 template <int DIM, int SRC, bool HEUN, int W0M>
 hipError_t launch_stage_t(const amx_fv_ctx* c, const StageArgs& a, int nspan, hipStream_t st) {
+  if (c->cfg.riemann == AMX_LAX_FRIEDRICHS) {
+    // the grid-wide alpha of every slot's stage input, reduced before the stage
+    hipLaunchKernelGGL(k_lf_clear, dim3(nspan), dim3(NSLOT), 0, st, a.ctl, a.slot0);
+    hipLaunchKernelGGL((k_lf_alpha<DIM, SRC>), dim3(LF_BLOCKS, nspan), dim3(256), 0, st, a);
+  }
   const dim3 grid(c->ntb * c->nta * c->ncx, nspan), block((TA_DEF + 1) * TB);
-  if (c->cfg.riemann == AMX_HLLC)
-    hipLaunchKernelGGL((k_stage<DIM, TA_DEF, SRC, HEUN, W0M, AMX_HLLC>), grid, block, 0, st, a);
+  if (c->cfg.limiter == AMX_MINMOD)
+    launch_stage_rs<DIM, SRC, HEUN, W0M, false>(c, a, grid, block, st);
   else
-    hipLaunchKernelGGL((k_stage<DIM, TA_DEF, SRC, HEUN, W0M, AMX_HLL>), grid, block, 0, st, a);
+    launch_stage_rs<DIM, SRC, HEUN, W0M, true>(c, a, grid, block, st);
   return hipGetLastError();
 }
 
@@ -1386,6 +1546,8 @@ hipError_t launch_stage(const amx_fv_ctx* c, const StageArgs& a, int nspan, hipS
 StageArgs stage_base(const amx_fv_ctx* c, float dt_scale) {
   StageArgs a;
   memset(&a, 0, sizeof(a));
+  a.limiter = c->cfg.limiter;
+  a.dxf = (float)c->cfg.grid_spacing;
   a.ctl = c->ctl;
   a.phs = c->dph;
   a.sstride = c->sstride;
@@ -1956,6 +2118,8 @@ extern "C" amx_status amx_fv_adjoint_step(amx_fv_ctx* c, const float* state, con
   if (!c || !state || !d_dt || !lam_out || !lam_in) return fail(AMX_ERR_INVALID, "null argument%s");
   if (c->own_state) return fail(AMX_ERR_INVALID, "a pool context has no reverse mode%s");
   if (c->cfg.mhd) return fail(AMX_ERR_UNSUPPORTED, "reverse mode covers the hydro step only%s");
+  if ((c->cfg.riemann != AMX_HLL && c->cfg.riemann != AMX_HLLC) || c->cfg.limiter != AMX_MINMOD)
+    return fail(AMX_ERR_UNSUPPORTED, "reverse mode covers the HLL / HLLC fluxes with the minmod limiter%s");
   if (lam_in == lam_out || lam_in == state) return fail(AMX_ERR_INVALID, "lam_in must not alias its inputs%s");
   const int NV = c->NV, dim = c->cfg.dim;
   if (!c->adj) {

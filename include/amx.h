@@ -36,6 +36,14 @@ typedef enum amx_status {
 /* Riemann solvers (simulation_config.py:41-46 values). */
 #define AMX_HLL 0
 #define AMX_HLLC 1
+#define AMX_HLLC_LM 2           /* low-Mach HLLC (hll.py:206-221) */
+#define AMX_LAX_FRIEDRICHS 3    /* grid-wide alpha per direction (_lax_friedrichs.py:21-78) */
+/* Slope limiters (simulation_config.py:29-34 values; VAN_ALBADA_PP = 5 is not implemented). */
+#define AMX_MINMOD 0
+#define AMX_OSHER 1
+#define AMX_DOUBLE_MINMOD 2
+#define AMX_SUPERBEE 3
+#define AMX_VAN_ALBADA 4
 /* Magnetic sub-step integrators (simulation_config.py:88-89 values). */
 #define AMX_IMPLICIT_MIDPOINT 0
 #define AMX_IMPLICIT_EULER 1
@@ -47,12 +55,13 @@ typedef struct amx_fv_config {
   int dim;                        /* 2 or 3 (config.dimensionality) */
   int nx, ny, nz;                 /* cells per axis; nz = 1 in 2D */
   int mhd;                        /* config.mhd */
-  int riemann;                    /* AMX_HLL / AMX_HLLC (config.riemann_solver) */
+  int riemann;                    /* AMX_HLL / AMX_HLLC / AMX_HLLC_LM / AMX_LAX_FRIEDRICHS (config.riemann_solver) */
   int magnetic_integrator;        /* config.fv_magnetic_integrator */
   int max_fixed_point_iterations; /* 1000 in the reference (_magnetic_field_update.py:141) */
   float gamma;                    /* params.gamma */
   float C_cfl;                    /* params.C_cfl */
   float fixed_point_tol;          /* 1e-5 for SINGLE_PRECISION (_magnetic_field_update.py:134-139) */
+  int limiter;                    /* AMX_MINMOD ... AMX_VAN_ALBADA (config.limiter) */
   double grid_spacing;            /* config.grid_spacing = box_size / num_cells */
 } amx_fv_config;
 

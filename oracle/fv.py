@@ -38,6 +38,7 @@ class OracleConfig(ctypes.Structure):
         ("num_timesteps", ctypes.c_int),
         ("exact_end_time", ctypes.c_int),
         ("max_iter", ctypes.c_int),
+        ("limiter", ctypes.c_int),
         ("gamma", ctypes.c_double),
         ("dx", ctypes.c_double),
         ("C_cfl", ctypes.c_double),
@@ -77,8 +78,9 @@ def _load():
                 ctypes.POINTER(OracleConfig), P, P, ctypes.c_double]
             getattr(lib, f"oracle_magnetic_update_{sfx}").restype = ctypes.c_int
             getattr(lib, f"oracle_riemann_{sfx}").argtypes = [
-                ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_long, P, P, P]
+                ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_long, P, P, P]
             getattr(lib, f"oracle_minmod_{sfx}").argtypes = [ctypes.c_long, P, P, P]
+            getattr(lib, f"oracle_limit_{sfx}").argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_long, P, P, P]
         _lib = lib
     return _lib
 
@@ -94,14 +96,14 @@ def _sfx(dtype) -> str:
 
 def make_config(state_shape, dim, *, mhd=True, riemann=0, euler=0, gamma=5.0 / 3.0,
                 box_size=1.0, C_cfl=0.4, t_end=0.2, tol=1e-5, fixed_timestep=False,
-                num_timesteps=1000, exact_end_time=True, max_iter=1000) -> OracleConfig:
+                num_timesteps=1000, exact_end_time=True, max_iter=1000, limiter=0) -> OracleConfig:
     n = state_shape[-1]
     sh = list(state_shape[1:]) + [1]
     return OracleConfig(
         dim=dim, nx=sh[0], ny=sh[1], nz=sh[2] if dim == 3 else 1, mhd=int(mhd),
         riemann=int(riemann), euler=int(euler), fixed_timestep=int(fixed_timestep),
         num_timesteps=int(num_timesteps), exact_end_time=int(exact_end_time),
-        max_iter=int(max_iter), gamma=float(gamma), dx=float(box_size) / n,
+        max_iter=int(max_iter), limiter=int(limiter), gamma=float(gamma), dx=float(box_size) / n,
         C_cfl=float(C_cfl), t_end=float(t_end), tol=float(tol))
 
 
@@ -157,15 +159,27 @@ def magnetic_update(B: np.ndarray, gas: np.ndarray, oc: OracleConfig, dt: float)
     return b, g, int(k)
 
 
-def riemann(wl: np.ndarray, wr: np.ndarray, dim: int, axis: int, gamma: float, solver: int = 0):
-    """Interface fluxes for (ng, n) left/right primitive states; axis 1-based."""
+def riemann(wl: np.ndarray, wr: np.ndarray, dim: int, axis: int, gamma: float, solver: int = 0,
+            alpha: float = 0.0):
+    """Interface fluxes for (ng, n) left/right primitive states; axis 1-based;
+    alpha is the Lax-Friedrichs dissipation speed (solver 3)."""
     lib = _load()
     a = np.ascontiguousarray(wl)
     b = np.ascontiguousarray(wr, dtype=a.dtype)
     f = np.empty_like(a)
-    getattr(lib, f"oracle_riemann_{_sfx(a.dtype)}")(dim, solver, axis, float(gamma), a.shape[1],
+    getattr(lib, f"oracle_riemann_{_sfx(a.dtype)}")(dim, solver, axis, float(gamma), float(alpha), a.shape[1],
                                                     _ptr(a), _ptr(b), _ptr(f))
     return f
+
+
+def limit(lim: int, a: np.ndarray, b: np.ndarray, dx: float) -> np.ndarray:
+    """Limited gradient from backward / forward gradients a, b (limiter id lim)."""
+    lib = _load()
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b, dtype=a.dtype)
+    out = np.empty_like(a)
+    getattr(lib, f"oracle_limit_{_sfx(a.dtype)}")(int(lim), float(dx), a.size, _ptr(a), _ptr(b), _ptr(out))
+    return out
 
 
 def minmod(a: np.ndarray, b: np.ndarray) -> np.ndarray:

@@ -9,10 +9,12 @@
 //   * conserved_state_from_primitive / primitive_state_from_conserved
 //       astronomix/_fluid_equations/_equations.py:23-156
 //   * _euler_flux                      astronomix/_fluid_equations/_fluxes.py:22-93
-//   * _minmod                          astronomix/_finite_volume/_state_evolution/limiters.py:13-15
-//   * _calculate_limited_gradients     .../_state_evolution/limited_gradients.py:21-88 (MINMOD, CARTESIAN)
+//   * _minmod, _double_minmod, _superbee, _van_albada_limiter, Osher
+//                                      astronomix/_finite_volume/_state_evolution/limiters.py:13-64
+//   * _calculate_limited_gradients     .../_state_evolution/limited_gradients.py:21-88 (CARTESIAN)
 //   * _reconstruct_at_interface_unsplit_single  .../_state_evolution/reconstruction.py:397-431
-//   * _hll_solver / _hllc_solver       astronomix/_finite_volume/_riemann_solver/hll.py:33-261
+//   * _hll_solver / _hllc_solver (+ HLLC-LM)  astronomix/_finite_volume/_riemann_solver/hll.py:33-261
+//   * _lax_friedrichs_solver           astronomix/_finite_volume/_riemann_solver/_lax_friedrichs.py:21-78
 //   * _evolve_gas_state_unsplit_inner  .../_state_evolution/evolve_state.py:338-414
 //   * _evolve_gas_state_unsplit (RK2)  .../_state_evolution/evolve_state.py:418-488
 //   * magnetic_update (Pang-Wu)        astronomix/_finite_volume/_magnetic_update/_magnetic_field_update.py:36-242
@@ -41,7 +43,8 @@
 
 namespace {
 
-enum { RS_HLL = 0, RS_HLLC = 1 };
+enum { RS_HLL = 0, RS_HLLC = 1, RS_HLLC_LM = 2, RS_LF = 3 };
+enum { LIM_MINMOD = 0, LIM_OSHER = 1, LIM_DOUBLE_MINMOD = 2, LIM_SUPERBEE = 3, LIM_VAN_ALBADA = 4 };
 
 template <typename T>
 struct Grid {
@@ -52,7 +55,8 @@ struct Grid {
   T gamma;           // params.gamma in the working precision
   T dx;              // config.grid_spacing in the working precision
   T inv_dx;          // (1 / config.grid_spacing) rounded once from double
-  int riemann;       // RS_HLL or RS_HLLC
+  int riemann;       // RS_*
+  int limiter;       // LIM_*
   int ng() const { return dim + 2; }        // number of gas variables
   int pidx() const { return dim + 1; }      // pressure / energy row
   long shift(long c, int axis, int s) const {
@@ -68,7 +72,7 @@ struct Grid {
 };
 
 template <typename T>
-Grid<T> make_grid(int dim, int nx, int ny, int nz, double gamma, double dx, int riemann) {
+Grid<T> make_grid(int dim, int nx, int ny, int nz, double gamma, double dx, int riemann, int limiter = 0) {
   Grid<T> g;
   g.dim = dim;
   g.n[0] = nx;
@@ -86,6 +90,7 @@ Grid<T> make_grid(int dim, int nx, int ny, int nz, double gamma, double dx, int 
   g.dx = (T)dx;
   g.inv_dx = (T)(1.0 / dx);
   g.riemann = riemann;
+  g.limiter = limiter;
   return g;
 }
 
@@ -98,6 +103,42 @@ inline T sgn(T a) {
 template <typename T>
 inline T minmod(T a, T b) {
   return (T(0.5) * (sgn(a) + sgn(b))) * std::min(std::fabs(a), std::fabs(b));
+}
+
+// limiters.py:18-25
+template <typename T>
+inline T minmod3(T a, T b, T c) {
+  const bool same = (sgn(a) == sgn(b)) && (sgn(b) == sgn(c));
+  return same ? sgn(a) * std::min(std::fabs(a), std::min(std::fabs(b), std::fabs(c))) : T(0);
+}
+
+// limiters.py:35-37
+template <typename T>
+inline T maxmod(T a, T b) {
+  return (T(0.5) * (sgn(a) + sgn(b))) * std::max(std::fabs(a), std::fabs(b));
+}
+
+// Limited gradient from the backward / forward gradients a, b
+// (limited_gradients.py:58-86, limiters.py:13-64); dx for van Albada's eps.
+template <typename T>
+inline T limit(int lim, T a, T b, T dx) {
+  switch (lim) {
+    case LIM_SUPERBEE:  // limiters.py:40-44
+      return a * b > 0 ? minmod(maxmod(a, b), minmod(T(2) * a, T(2) * b)) : T(0);
+    case LIM_DOUBLE_MINMOD:  // limiters.py:28-32
+      return a * b > 0 ? minmod3((a + b) / T(2), T(2) * a, T(2) * b) : T(0);
+    case LIM_VAN_ALBADA: {  // limiters.py:48-64, eps = 3 dx
+      const T eps = T(3) * dx, eps2 = T(2) * eps;
+      return ((b * b + eps) * a + (a * a + eps) * b) / ((b * b + a * a) + eps2);
+    }
+    case LIM_OSHER: {  // limited_gradients.py:72-84, beta = 1.3
+      const T eps = T(1e-11);
+      const T r = std::fabs(a) > eps ? b / (a + eps) : T(0);
+      return std::max(T(0), std::min(T(1.3), r)) * a;
+    }
+    default:
+      return minmod(a, b);
+  }
 }
 
 // _equations.py:164-199 (get_absolute_velocity) -- 3D adds 1e-20, 2D does not
@@ -176,9 +217,10 @@ inline void hll_flux(const T* wl, const T* wr, T* f, int dim, int axis, T gamma)
     f[v] = ((sp * fl[v] - sm * fr[v]) + (sm * sp) * (cr_[v] - cl_[v])) / (sp - sm);
 }
 
-// hll.py:113-261 (plain HLLC: no LM / low-Mach dissipation control)
+// hll.py:113-261: HLLC, and with lm the HLLC-LM variant of hll.py:206-221
+// (no low-Mach dissipation control: that belongs to AM-HLLC)
 template <typename T>
-inline void hllc_flux(const T* wl, const T* wr, T* f, int dim, int axis, T gamma) {
+inline void hllc_flux(const T* wl, const T* wr, T* f, int dim, int axis, T gamma, bool lm = false) {
   const int P = dim + 1;
   const int ng = dim + 2;
   const T rl = wl[0], ul = wl[axis], pl = wl[P];
@@ -214,14 +256,47 @@ inline void hllc_flux(const T* wl, const T* wr, T* f, int dim, int axis, T gamma
     USR[v] = (USR[v] * (SR - ur)) / (SR - s_star);
   }
   const T as = std::fabs(s_star);
+  T SLd = SL, SRd = SR;
+  if (lm) {
+    const T ma = std::max(std::fabs(ul / cl), std::fabs(ur / cr));
+    const T phi = std::sin((std::min(T(1), ma / T(0.1)) * T(3.14159265358979323846)) / T(2));
+    SLd = SL * phi;
+    SRd = SR * phi;
+  }
   for (int v = 0; v < ng; ++v) {
     const T bulk = T(0.5) * (FL[v] + FR[v]);
-    const T diss = T(0.5) * ((SL * (USL[v] - UL[v]) + as * (USL[v] - USR[v])) + SR * (USR[v] - UR[v]));
+    const T diss = T(0.5) * ((SLd * (USL[v] - UL[v]) + as * (USL[v] - USR[v])) + SRd * (USR[v] - UR[v]));
     T fs = bulk + diss;
     T out = SL >= 0 ? FL[v] : fs;
     out = SR <= 0 ? FR[v] : out;
     f[v] = out;
   }
+}
+
+// _lax_friedrichs.py:21-78: 0.5 (F_L + F_R) - 0.5 alpha (U_R - U_L)
+template <typename T>
+inline void lf_flux(const T* wl, const T* wr, T* f, int dim, int axis, T gamma, T alpha) {
+  const int ng = dim + 2;
+  T FL[5], FR[5], UL[5], UR[5];
+  euler_flux(wl, FL, dim, axis, gamma);
+  euler_flux(wr, FR, dim, axis, gamma);
+  cons_from_prim(wl, UL, dim, gamma);
+  cons_from_prim(wr, UR, dim, gamma);
+  for (int v = 0; v < ng; ++v) f[v] = T(0.5) * (FL[v] + FR[v]) - (T(0.5) * alpha) * (UR[v] - UL[v]);
+}
+
+// alpha of the LF solver along axis (1-based): max over cells of |u| + c (_lax_friedrichs.py:61-66)
+template <typename T>
+T lf_alpha(const Grid<T>& g, const T* w, int axis) {
+  const long nc = g.ncell;
+  const int P = g.pidx();
+  T m = 0;
+  for (long c = 0; c < nc; ++c) {
+    const T cs = std::sqrt((g.gamma * w[P * nc + c]) / w[c]);
+    const T v = std::fabs(w[axis * nc + c]) + cs;
+    if (v != v || v > m) m = v;
+  }
+  return m;
 }
 
 // One forward-Euler unsplit stage, evolve_state.py:338-414.
@@ -240,6 +315,7 @@ void gas_stage(const Grid<T>& g, const T* w, T* out, T dt) {
   }
   std::vector<T> F((size_t)ng * nc);
   for (int axis = 1; axis <= g.dim; ++axis) {
+    const T alpha = g.riemann == RS_LF ? lf_alpha(g, w, axis) : T(0);
     // interface fluxes F[i] at i-1/2 (reconstruction.py:412-429, hll.py)
 #pragma omp parallel for schedule(static)
     for (long c = 0; c < nc; ++c) {
@@ -250,15 +326,17 @@ void gas_stage(const Grid<T>& g, const T* w, T* out, T dt) {
       for (int v = 0; v < ng; ++v) {
         const T a = w[v * nc + cm2], b = w[v * nc + cm1], cc = w[v * nc + c], d = w[v * nc + cp1];
         // limited_gradients.py:50-60 ; reconstruction.py:409-429
-        const T g_m1 = minmod((b - a) / g.dx, (cc - b) / g.dx);
-        const T g_0 = minmod((cc - b) / g.dx, (d - cc) / g.dx);
+        const T g_m1 = limit(g.limiter, (b - a) / g.dx, (cc - b) / g.dx, g.dx);
+        const T g_0 = limit(g.limiter, (cc - b) / g.dx, (d - cc) / g.dx, g.dx);
         const T d_m1 = (g_m1 * g.dx) / T(2);
         const T d_0 = (g_0 * g.dx) / T(2);
         wl[v] = b + d_m1;
         wr[v] = cc - d_0;
       }
-      if (g.riemann == RS_HLLC)
-        hllc_flux(wl, wr, f, g.dim, axis, g.gamma);
+      if (g.riemann == RS_HLLC || g.riemann == RS_HLLC_LM)
+        hllc_flux(wl, wr, f, g.dim, axis, g.gamma, g.riemann == RS_HLLC_LM);
+      else if (g.riemann == RS_LF)
+        lf_flux(wl, wr, f, g.dim, axis, g.gamma, alpha);
       else
         hll_flux(wl, wr, f, g.dim, axis, g.gamma);
       for (int v = 0; v < ng; ++v) F[v * nc + c] = f[v];
@@ -478,19 +556,20 @@ struct OracleConfig {
   int dim;
   int nx, ny, nz;
   int mhd;
-  int riemann;        // 0 HLL, 1 HLLC
+  int riemann;        // 0 HLL, 1 HLLC, 2 HLLC-LM, 3 Lax-Friedrichs
   int euler;          // fv_magnetic_integrator: 0 implicit midpoint, 1 implicit Euler
   int fixed_timestep;
   int num_timesteps;
   int exact_end_time;
   int max_iter;       // magnetic fixed-point cap (1000 in the reference)
+  int limiter;        // 0 minmod, 1 Osher, 2 double minmod, 3 superbee, 4 van Albada
   double gamma, dx, C_cfl, t_end, tol;
 };
 
 // loop of _time_integration (time_integration.py:629-801) without snapshots
 template <typename T>
 long time_integration(const OracleConfig& oc, T* state, long max_steps, double* t_out, long* iters_out) {
-  Grid<T> g = make_grid<T>(oc.dim, oc.nx, oc.ny, oc.nz, oc.gamma, oc.dx, oc.riemann);
+  Grid<T> g = make_grid<T>(oc.dim, oc.nx, oc.ny, oc.nz, oc.gamma, oc.dx, oc.riemann, oc.limiter);
   T t = 0;
   const T t_end = (T)oc.t_end;
   long steps = 0, iters = 0;
@@ -527,28 +606,28 @@ extern "C" {
     return time_integration<T>(*oc, state, max_steps, t_out, iters_out);                           \
   }                                                                                                \
   int oracle_evolve_state_fv_##SFX(const OracleConfig* oc, T* state, double dt) {                  \
-    Grid<T> g = make_grid<T>(oc->dim, oc->nx, oc->ny, oc->nz, oc->gamma, oc->dx, oc->riemann);     \
+    Grid<T> g = make_grid<T>(oc->dim, oc->nx, oc->ny, oc->nz, oc->gamma, oc->dx, oc->riemann, oc->limiter);     \
     return evolve_state_fv<T>(g, state, oc->mhd, (T)dt, (T)oc->tol, oc->max_iter, oc->euler);      \
   }                                                                                                \
   double oracle_cfl_dt_##SFX(const OracleConfig* oc, const T* gas) {                               \
-    Grid<T> g = make_grid<T>(oc->dim, oc->nx, oc->ny, oc->nz, oc->gamma, oc->dx, oc->riemann);     \
+    Grid<T> g = make_grid<T>(oc->dim, oc->nx, oc->ny, oc->nz, oc->gamma, oc->dx, oc->riemann, oc->limiter);     \
     return (double)cfl_dt<T>(g, gas, (T)oc->C_cfl);                                                \
   }                                                                                                \
   void oracle_gas_stage_##SFX(const OracleConfig* oc, const T* w, T* out, double dt) {             \
-    Grid<T> g = make_grid<T>(oc->dim, oc->nx, oc->ny, oc->nz, oc->gamma, oc->dx, oc->riemann);     \
+    Grid<T> g = make_grid<T>(oc->dim, oc->nx, oc->ny, oc->nz, oc->gamma, oc->dx, oc->riemann, oc->limiter);     \
     gas_stage<T>(g, w, out, (T)dt);                                                                \
   }                                                                                                \
   void oracle_gas_rk2_##SFX(const OracleConfig* oc, T* w, double dt) {                             \
-    Grid<T> g = make_grid<T>(oc->dim, oc->nx, oc->ny, oc->nz, oc->gamma, oc->dx, oc->riemann);     \
+    Grid<T> g = make_grid<T>(oc->dim, oc->nx, oc->ny, oc->nz, oc->gamma, oc->dx, oc->riemann, oc->limiter);     \
     gas_rk2<T>(g, w, (T)dt);                                                                       \
   }                                                                                                \
   int oracle_magnetic_update_##SFX(const OracleConfig* oc, T* B, T* gas, double dt) {              \
-    Grid<T> g = make_grid<T>(oc->dim, oc->nx, oc->ny, oc->nz, oc->gamma, oc->dx, oc->riemann);     \
+    Grid<T> g = make_grid<T>(oc->dim, oc->nx, oc->ny, oc->nz, oc->gamma, oc->dx, oc->riemann, oc->limiter);     \
     return magnetic_update<T>(g, B, gas, (T)dt, (T)oc->tol, oc->max_iter, oc->euler);              \
   }                                                                                                \
   /* interface flux for n independent (wl, wr) pairs laid out (ng, n) */                          \
-  void oracle_riemann_##SFX(int dim, int riemann, int axis, double gamma, long n, const T* wl,      \
-                            const T* wr, T* f) {                                                   \
+  void oracle_riemann_##SFX(int dim, int riemann, int axis, double gamma, double alpha, long n,     \
+                            const T* wl, const T* wr, T* f) {                                      \
     const int ng = dim + 2;                                                                        \
     for (long i = 0; i < n; ++i) {                                                                 \
       T a[5], b[5], o[5];                                                                          \
@@ -556,8 +635,10 @@ extern "C" {
         a[v] = wl[v * n + i];                                                                      \
         b[v] = wr[v * n + i];                                                                      \
       }                                                                                            \
-      if (riemann == RS_HLLC)                                                                      \
-        hllc_flux<T>(a, b, o, dim, axis, (T)gamma);                                                \
+      if (riemann == RS_HLLC || riemann == RS_HLLC_LM)                                             \
+        hllc_flux<T>(a, b, o, dim, axis, (T)gamma, riemann == RS_HLLC_LM);                         \
+      else if (riemann == RS_LF)                                                                   \
+        lf_flux<T>(a, b, o, dim, axis, (T)gamma, (T)alpha);                                        \
       else                                                                                         \
         hll_flux<T>(a, b, o, dim, axis, (T)gamma);                                                 \
       for (int v = 0; v < ng; ++v) f[v * n + i] = o[v];                                            \
@@ -565,6 +646,10 @@ extern "C" {
   }                                                                                                \
   void oracle_minmod_##SFX(long n, const T* a, const T* b, T* out) {                               \
     for (long i = 0; i < n; ++i) out[i] = minmod<T>(a[i], b[i]);                                   \
+  }                                                                                                \
+  /* limited gradient of n (backward, forward) gradient pairs */                                  \
+  void oracle_limit_##SFX(int lim, double dx, long n, const T* a, const T* b, T* out) {            \
+    for (long i = 0; i < n; ++i) out[i] = limit<T>(lim, a[i], b[i], (T)dx);                        \
   }
 
 AMX_ORACLE_DEFS(float, f32)
