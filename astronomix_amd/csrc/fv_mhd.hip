@@ -1242,7 +1242,7 @@ constexpr int TA_DEF = AMX_STAGE_TA;
 // covers the tile and a two-cell ring, (TA + 4) x (TB + 4) cells rounded up
 // to whole 256-thread groups
 #ifndef AMX_MAG_TA
-#define AMX_MAG_TA 7
+#define AMX_MAG_TA 11  // (11 + 4) x 68 = 1020 cells: one 16-wave block, 4 waves per SIMD at <= 128 VGPRs
 #endif
 #ifndef AMX_MAG_TB
 #define AMX_MAG_TB 64
