@@ -715,18 +715,18 @@ struct MagRaw {
   float Bk[3], vk[3], B0[3], v0[3];
 };
 
-__device__ __forceinline__ void mag_load(const MagArgs& a, long c, MagRaw& m) {
+__device__ __forceinline__ void mag_load(const MagArgs& a, unsigned c, MagRaw& m) {
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
-    m.B0[i] = a.B0[i][c];
-    m.Bk[i] = a.Bk[i][c];
+    m.B0[i] = ld32(a.B0[i], c);
+    m.Bk[i] = ld32(a.Bk[i], c);
   }
-  m.v0[0] = a.v0[0][c];
-  m.v0[1] = a.v0[1][c];
-  m.v0[2] = a.v0[2] ? a.v0[2][c] : 0.f;
-  m.vk[0] = a.vk[0][c];
-  m.vk[1] = a.vk[1][c];
-  m.vk[2] = a.vk[2] ? a.vk[2][c] : 0.f;
+  m.v0[0] = ld32(a.v0[0], c);
+  m.v0[1] = ld32(a.v0[1], c);
+  m.v0[2] = a.v0[2] ? ld32(a.v0[2], c) : 0.f;
+  m.vk[0] = ld32(a.vk[0], c);
+  m.vk[1] = ld32(a.vk[1], c);
+  m.vk[2] = a.vk[2] ? ld32(a.vk[2], c) : 0.f;
 }
 
 // (avgB, E = avgB x avgv) of iterate (Bk, vk) (_magnetic_field_update.py:84,149-154; _vector_maths.py:5-10)
@@ -859,14 +859,15 @@ __global__ __launch_bounds__(((TA + 4) * (TBM + 4) + 255) / 256 * 256) void k_ma
   const int ita = rest % a.nta;
   const int icx = rest / a.nta;
   const int a0 = ita * TA, b0 = itb * TBM;
-  const long sa = a.nb, sx = (long)a.na * a.nb;
+  // 32-bit cell offsets (a slot has < 2^30 cells)
+  const unsigned sa = a.nb, sx = (unsigned)a.na * a.nb;
   const bool has = tid < NC;
   const int r = has ? tid / WX - 2 : 0, c = has ? tid % WX - 2 : 0;
   const bool ring2 = r == -2 || r == TA + 1 || c == -2 || c == TBM + 1;
   const bool inner = has && !ring2;                       // first iterate here
   const bool tile = inner && r >= 0 && r < TA && c >= 0 && c < TBM;
   const bool own = tile && (a0 + r < a.na) && (b0 + c < a.nb);
-  const long col = (long)wrapi(a0 + r, a.na) * sa + wrapi(b0 + c, a.nb);
+  const unsigned col = (unsigned)wrapi(a0 + r, a.na) * sa + wrapi(b0 + c, a.nb);
   const int o = (r + 2) * WX + (c + 2);
   const int o1 = (r + 1) * W1 + (c + 1);
   const float dt = C->dt;
@@ -884,7 +885,7 @@ __global__ __launch_bounds__(((TA + 4) * (TBM + 4) + 255) / 256 * 256) void k_ma
 #pragma unroll
       for (int i = 0; i < 6; ++i) sK[0][i][o] = d[i];
     }
-    if (inner) rho = m.rho[col];
+    if (inner) rho = ld32(m.rho, col);
     __syncthreads();
     float B1[3], v1[3];
     if (inner) {
@@ -896,8 +897,8 @@ __global__ __launch_bounds__(((TA + 4) * (TBM + 4) + 255) / 256 * 256) void k_ma
       if (own) {
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
-          m.Bo[0][i][col] = B1[i];
-          m.vo[0][i][col] = v1[i];
+          st32(m.Bo[0][i], col, B1[i]);
+          st32(m.vo[0][i], col, v1[i]);
         }
         const float rr = mag_resid(rc.Bk, rc.vk, B1, v1);
         res0 = (rr != rr || rr > res0) ? rr : res0;
@@ -918,8 +919,8 @@ __global__ __launch_bounds__(((TA + 4) * (TBM + 4) + 255) / 256 * 256) void k_ma
       mag_apply(e1, cE, cB, rc.B0, rc.v0, rho, dt, B2, v2);
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
-        m.Bo[1][i][col] = B2[i];
-        m.vo[1][i][col] = v2[i];
+        st32(m.Bo[1][i], col, B2[i]);
+        st32(m.vo[1][i], col, v2[i]);
       }
       const float rr = mag_resid(B1, v1, B2, v2);
       res1 = (rr != rr || rr > res1) ? rr : res1;
@@ -931,16 +932,16 @@ __global__ __launch_bounds__(((TA + 4) * (TBM + 4) + 255) / 256 * 256) void k_ma
     float rho_c = 1.f, rho_n = 1.f;
     if (has) {
       MagRaw t;
-      mag_load(m, (long)wrapi(x0 - 2, a.nx) * sx + col, t);
+      mag_load(m, (unsigned)wrapi(x0 - 2, a.nx) * sx + col, t);
       mag_derive(t, a.euler, dm);
-      mag_load(m, (long)wrapi(x0 - 1, a.nx) * sx + col, rc);
+      mag_load(m, (unsigned)wrapi(x0 - 1, a.nx) * sx + col, rc);
       mag_derive(rc, a.euler, dc);
-      mag_load(m, (long)x0 * sx + col, rn);
+      mag_load(m, (unsigned)x0 * sx + col, rn);
       mag_derive(rn, a.euler, dp);
     }
     if (inner) {
-      rho_c = m.rho[(long)wrapi(x0 - 1, a.nx) * sx + col];
-      rho_n = m.rho[(long)x0 * sx + col];
+      rho_c = ld32(m.rho, (unsigned)wrapi(x0 - 1, a.nx) * sx + col);
+      rho_n = ld32(m.rho, (unsigned)x0 * sx + col);
     }
     // second-iterate carries of the own column (tile threads): derived first
     // iterate at x-2, x-1; first iterate and fixed-point inputs at x-1
@@ -954,9 +955,9 @@ __global__ __launch_bounds__(((TA + 4) * (TBM + 4) + 255) / 256 * 256) void k_ma
       // prefetch x + 2
       MagRaw rnn;
       float rho_nn = 1.f;
-      const long pnn = (long)wrapi(x + 2, a.nx) * sx + col;
+      const unsigned pnn = (unsigned)wrapi(x + 2, a.nx) * sx + col;
       if (has) mag_load(m, pnn, rnn);
-      if (inner) rho_nn = m.rho[pnn];
+      if (inner) rho_nn = ld32(m.rho, pnn);
       if (has) {
 #pragma unroll
         for (int i = 0; i < 6; ++i) sK[bx][i][o] = dc[i];
@@ -970,11 +971,11 @@ __global__ __launch_bounds__(((TA + 4) * (TBM + 4) + 255) / 256 * 256) void k_ma
         mag_curls<3, WX>(SK, o, dm, dp, idx, cE, cB);
         mag_apply(dc, cE, cB, rc.B0, rc.v0, rho_c, dt, B1, v1);
         if (own && x >= x0 && x < x1) {
-          const long cell = (long)x * sx + col;
+          const unsigned cell = (unsigned)x * sx + col;
 #pragma unroll
           for (int i = 0; i < 3; ++i) {
-            m.Bo[0][i][cell] = B1[i];
-            m.vo[0][i][cell] = v1[i];
+            st32(m.Bo[0][i], cell, B1[i]);
+            st32(m.vo[0][i], cell, v1[i]);
           }
           const float rr = mag_resid(rc.Bk, rc.vk, B1, v1);
           res0 = (rr != rr || rr > res0) ? rr : res0;
@@ -990,11 +991,11 @@ __global__ __launch_bounds__(((TA + 4) * (TBM + 4) + 255) / 256 * 256) void k_ma
         for (int i = 0; i < 6; ++i) SK1[i] = sK1[bx ^ 1][i];
         mag_curls<3, W1>(SK1, o1, e1m, e1, idx, cE, cB);
         mag_apply(e1c, cE, cB, B0p, v0p, rhop, dt, B2, v2);
-        const long cell = (long)(x - 1) * sx + col;
+        const unsigned cell = (unsigned)(x - 1) * sx + col;
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
-          m.Bo[1][i][cell] = B2[i];
-          m.vo[1][i][cell] = v2[i];
+          st32(m.Bo[1][i], cell, B2[i]);
+          st32(m.vo[1][i], cell, v2[i]);
         }
         const float rr = mag_resid(Bp, vp, B2, v2);
         res1 = (rr != rr || rr > res1) ? rr : res1;
