@@ -10,6 +10,7 @@ from __future__ import annotations
 
 from astronomix_amd.option_classes.simulation_config import (
     ALL_PERIODIC,
+    AM_HLLC,
     CARTESIAN,
     DOUBLE_MINMOD,
     DOUBLE_PRECISION,
@@ -17,6 +18,7 @@ from astronomix_amd.option_classes.simulation_config import (
     HLL,
     HLLC,
     HLLC_LM,
+    HYBRID_HLLC,
     IMPLICIT_EULER,
     IMPLICIT_MIDPOINT,
     LAX_FRIEDRICHS,
@@ -36,7 +38,7 @@ def unsupported_reason(config, registered_variables=None):
 
     The implemented path is the unsplit SSP-RK2 finite-volume scheme with the
     minmod, Osher, double-minmod, superbee or van Albada limiter, HLL, HLLC,
-    HLLC-LM or Lax-Friedrichs fluxes, periodic boundaries on every axis, 2D
+    HLLC-LM, hybrid / AM HLLC or Lax-Friedrichs fluxes, periodic boundaries on every axis, 2D
     or 3D, hydro or MHD (Pang-Wu magnetic sub-step), float32, no physics
     modules.
     """
@@ -55,8 +57,8 @@ def unsupported_reason(config, registered_variables=None):
         return "only the RK2_SSP integrator is implemented for the unsplit scheme"
     if c.limiter not in (MINMOD, OSHER, DOUBLE_MINMOD, SUPERBEE, VAN_ALBADA):
         return "the VAN_ALBADA_PP limiter is not implemented"
-    if c.riemann_solver not in (HLL, HLLC, HLLC_LM, LAX_FRIEDRICHS):
-        return "the HYBRID_HLLC and AM_HLLC Riemann solvers are not implemented"
+    if c.riemann_solver not in (HLL, HLLC, HLLC_LM, LAX_FRIEDRICHS, HYBRID_HLLC, AM_HLLC):
+        return "unknown Riemann solver"
     if c.fv_magnetic_integrator not in (IMPLICIT_MIDPOINT, IMPLICIT_EULER):
         return "unknown fv_magnetic_integrator"
     if c.first_order_fallback:

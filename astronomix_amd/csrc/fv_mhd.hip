@@ -187,10 +187,12 @@ __device__ __forceinline__ void flux_hll(const float* wl, const float* wr, float
 }
 
 // HLLC interface flux, central form (_finite_volume/_riemann_solver/hll.py:113-261);
-// LM: the low-Mach variant HLLC-LM, whose dissipation takes S_L, S_R scaled
-// by phi = sin(min(1, Ma / 0.1) pi / 2) (hll.py:206-221).
-template <int DIM, int AX, bool LM>
-__device__ __forceinline__ void flux_hllc(const float* wl, const float* wr, float* f, const Phys& ph) {
+// lm: the low-Mach variant HLLC-LM, whose dissipation takes S_L, S_R scaled
+// by phi = sin(min(1, Ma / 0.1) pi / 2) (hll.py:206-221); LMD: low-Mach
+// dissipation control, the velocity rows of the dissipation scaled by
+// min(1, max(|v_L| / c_L, |v_R| / c_R)) (hll.py:230-254).
+template <int DIM, int AX, bool LMD>
+__device__ __forceinline__ void flux_hllc(const float* wl, const float* wr, float* f, const Phys& ph, bool lm) {
   constexpr int P = DIM + 1, NV = DIM + 2;
   const float rl = wl[0], ul = wl[AX], pl = wl[P];
   const float rr = wr[0], ur = wr[AX], pr = wr[P];
@@ -229,11 +231,18 @@ __device__ __forceinline__ void flux_hllc(const float* wl, const float* wr, floa
   const float fr = (SR - ur) * frcp(SR - s_star);
   const float as = fabsf(s_star);
   float SLd = SL, SRd = SR;
-  if (LM) {
+  if (lm) {
     const float ma = fmaxf(fabsf(ul / cl), fabsf(ur / cr));
     const float phi = sinf((fminf(1.f, ma / 0.1f) * 3.14159265358979f) / 2.f);
     SLd = SL * phi;
     SRd = SR * phi;
+  }
+  float fmd = 1.f;
+  if (LMD) {
+    // get_absolute_velocity (_equations.py:165-197): +1e-20 in 3D only
+    const float al = sqrtf(DIM == 3 ? vsq<DIM>(wl) + 1e-20f : vsq<DIM>(wl));
+    const float ar = sqrtf(DIM == 3 ? vsq<DIM>(wr) + 1e-20f : vsq<DIM>(wr));
+    fmd = fminf(1.f, fmaxf(fabsf(al / cl), fabsf(ar / cr)));
   }
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
@@ -248,7 +257,9 @@ __device__ __forceinline__ void flux_hllc(const float* wl, const float* wr, floa
     }
     usl *= fl;
     usr *= fr;
-    const float fs = 0.5f * (FL[v] + FR[v]) + 0.5f * ((SLd * (usl - UL[v]) + as * (usl - usr)) + SRd * (usr - UR[v]));
+    float diss = 0.5f * ((SLd * (usl - UL[v]) + as * (usl - usr)) + SRd * (usr - UR[v]));
+    if (LMD && v >= 1 && v <= DIM) diss = fmd * diss;
+    const float fs = 0.5f * (FL[v] + FR[v]) + diss;
     float out = SL >= 0.f ? FL[v] : fs;
     f[v] = SR <= 0.f ? FR[v] : out;
   }
@@ -280,12 +291,19 @@ __device__ __forceinline__ void flux_lf(const float* wl, const float* wr, float*
 }
 
 // Riemann solver ids (simulation_config.py:41-46 values).
+// gsw: the compression switch of the hybrid / AM-HLLC solvers at the cell
+// right of the interface (HLLC-LM where it is set, hll.py:297-326).
 template <int DIM, int AX, int RS>
-__device__ __forceinline__ void riemann(const float* wl, const float* wr, float* f, const Phys& ph, float alpha) {
+__device__ __forceinline__ void riemann(const float* wl, const float* wr, float* f, const Phys& ph, float alpha,
+                                        bool gsw) {
   if (RS == AMX_HLLC)
-    flux_hllc<DIM, AX, false>(wl, wr, f, ph);
+    flux_hllc<DIM, AX, false>(wl, wr, f, ph, false);
   else if (RS == AMX_HLLC_LM)
-    flux_hllc<DIM, AX, true>(wl, wr, f, ph);
+    flux_hllc<DIM, AX, false>(wl, wr, f, ph, true);
+  else if (RS == AMX_HYBRID_HLLC)
+    flux_hllc<DIM, AX, false>(wl, wr, f, ph, gsw);
+  else if (RS == AMX_AM_HLLC)
+    flux_hllc<DIM, AX, true>(wl, wr, f, ph, gsw);
   else if (RS == AMX_LAX_FRIEDRICHS)
     flux_lf<DIM, AX>(wl, wr, f, ph, alpha);
   else
@@ -379,6 +397,8 @@ struct StageArgs {
   float dt_scale;        // 0.5: MHD half step dt/2; 1: hydro step
   int limiter;           // LIM_* (the kernels instantiated with LIMX read it)
   float dxf;             // float(grid_spacing): the limiters' gradient scale
+  unsigned char* amg;    // hybrid / AM-HLLC: compression switch per cell of the stage input
+  long gstride;          // bytes between consecutive slots' switch arrays
 };
 
 // One unsplit forward-Euler stage (evolve_state.py:338-414), optionally with
@@ -402,6 +422,8 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
   __shared__ float sW[NV][(TA + 4) * WW];
   __shared__ float sFA[NV][(TA + 1) * TB];
   __shared__ float sFB[NV][TA * (TB + 1)];
+  constexpr bool AMS = RS == AMX_HYBRID_HLLC || RS == AMX_AM_HLLC;
+  __shared__ unsigned char sG[AMS ? (TA + 4) * WW : 1];
 
   const int slot = a.slot0 + blockIdx.y;
   Ctl* const C = a.ctl + slot;
@@ -430,6 +452,7 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
   const int tid = threadIdx.x, ta = tid / TB, tb = tid % TB;
   const int lim = a.limiter;
   const float ldx = a.dxf;
+  const unsigned char* gsw = AMS ? a.amg + slot * a.gstride : nullptr;
   // Lax-Friedrichs: alpha per velocity row from the stage input's maxima
   float alf[3] = {0.f, 0.f, 0.f};
   if (RS == AMX_LAX_FRIEDRICHS) {
@@ -488,7 +511,7 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
         wl[v] = qm1[v] + slope<LIMX>(qm2[v], qm1[v], q0[v], lim, ldx);
         wr[v] = q0[v] - d0[v];
       }
-      riemann<DIM, 1, RS>(wl, wr, Fp, ph, alf[0]);
+      riemann<DIM, 1, RS>(wl, wr, Fp, ph, alf[0], AMS && gsw[(unsigned)x0 * sx + col]);
     } else {
       load_w<DIM, SRC>(src, col, q0);
     }
@@ -528,14 +551,17 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
           wr[v] = q1[v] - d1[v];
           d0[v] = d1[v];
         }
-        riemann<DIM, 1, RS>(wl, wr, Fn, ph, alf[0]);
+        riemann<DIM, 1, RS>(wl, wr, Fn, ph, alf[0],
+                            AMS && gsw[(unsigned)wrapi(x + 1, a.nx) * sx + col]);
       }
 #pragma unroll
       for (int v = 0; v < NV; ++v) sW[v][(ta + 2) * WW + tb + 2] = q0[v];
+      if (AMS) sG[(ta + 2) * WW + tb + 2] = gsw[plane + col];
     }
     if (h >= 0) {
 #pragma unroll
       for (int v = 0; v < NV; ++v) sW[v][(hr + 2) * WW + hc + 2] = hw[v];
+      if (AMS) sG[(hr + 2) * WW + hc + 2] = gsw[plane + hoff];
     }
     __syncthreads();
     // interface fluxes of the plane; limited slopes fused in (4-point
@@ -551,7 +577,7 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
         wl[v] = w1 + slope<LIMX>(w0, w1, w2, lim, ldx);
         wr[v] = w2 - slope<LIMX>(w1, w2, w3, lim, ldx);
       }
-      riemann<DIM, AXA, RS>(wl, wr, f, ph, alf[AXA - 1]);
+      riemann<DIM, AXA, RS>(wl, wr, f, ph, alf[AXA - 1], AMS && sG[o]);
 #pragma unroll
       for (int v = 0; v < NV; ++v) sFA[v][tid] = f[v];
     }
@@ -565,7 +591,7 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
         wl[v] = w1 + slope<LIMX>(w0, w1, w2, lim, ldx);
         wr[v] = w2 - slope<LIMX>(w1, w2, w3, lim, ldx);
       }
-      riemann<DIM, AXB, RS>(wl, wr, f, ph, alf[AXB - 1]);
+      riemann<DIM, AXB, RS>(wl, wr, f, ph, alf[AXB - 1], AMS && sG[o]);
 #pragma unroll
       for (int v = 0; v < NV; ++v) sFB[v][tid] = f[v];
     }
@@ -676,6 +702,57 @@ __global__ __launch_bounds__(256) void k_lf_alpha(const StageArgs a) {
     const unsigned b = block_max_u<4>(canon_bits(mx[k]), sred);
     if (threadIdx.x == 0 && b) atomicMax(&C->lfp[k][blockIdx.x & (NSLOT - 1)], b);
     __syncthreads();  // sred is reused by the next axis
+  }
+}
+
+// Compression switch of the hybrid / AM-HLLC solvers (hll.py:281-297): per
+// cell of the stage input, g = 1 where div v < -0.05 a / dx with the central
+// velocity differences (v[i+1] - v[i-1]) / (2 dx) summed over the axes and a
+// the sound speed, else 0.
+template <int DIM, int SRC>
+__global__ __launch_bounds__(256) void k_am_switch(const StageArgs a) {
+  constexpr int P = DIM + 1;
+  const int slot = a.slot0 + blockIdx.y;
+  Ctl* const C = a.ctl + slot;
+  if (!C->run) return;  // block-uniform
+  const long so = slot * a.sstride;
+  const int par = C->iters % 3;
+  const float* src[8];
+#pragma unroll
+  for (int v = 0; v < 8; ++v) src[v] = a.src[v] ? a.src[v] + so : nullptr;
+  if (SRC == SRC_MAGFIX) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) src[5 + k] = a.selv[par][k] + so;
+  }
+  // velocity rows of the stage input: the converged magnetic iterate's in MAGFIX
+  const float* vel[3];
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) vel[k] = SRC == SRC_MAGFIX ? src[5 + k] : src[1 + k];
+  unsigned char* g = a.amg + slot * a.gstride;
+  const Phys ph = a.phs[slot];
+  const float den = 2.f * a.dxf;  // float32(2 dx): 2 * d in the reference
+  const float thr_scale = -0.05f;
+  // axes of the state: 3D (nx, na, nb) = (x, y, z); 2D (na, nb) = (x, y)
+  const int n0 = DIM == 3 ? a.nx : a.na, n1 = DIM == 3 ? a.na : a.nb, n2 = DIM == 3 ? a.nb : 1;
+  const unsigned n = (unsigned)n0 * (unsigned)n1 * (unsigned)n2;
+  const unsigned s0 = (unsigned)n1 * (unsigned)n2, s1 = (unsigned)n2;
+  for (unsigned c = blockIdx.x * 256u + threadIdx.x; c < n; c += gridDim.x * 256u) {
+    const int i2 = (int)(c % (unsigned)n2);
+    const unsigned r = c / (unsigned)n2;
+    const int i1 = (int)(r % (unsigned)n1);
+    const int i0 = (int)(r / (unsigned)n1);
+    float w[DIM + 2];
+    load_w<DIM, SRC>(src, c, w);
+    const float cs = sqrtf((ph.gamma * w[P]) / w[0]);
+    const unsigned b0 = c - (unsigned)i0 * s0, b1 = c - (unsigned)i1 * s1, b2 = c - (unsigned)i2;
+    float div = ((ld32(vel[0], b0 + (unsigned)(i0 + 1 == n0 ? 0 : i0 + 1) * s0) -
+                  ld32(vel[0], b0 + (unsigned)(i0 == 0 ? n0 - 1 : i0 - 1) * s0)) / den);
+    div = div + (ld32(vel[1], b1 + (unsigned)(i1 + 1 == n1 ? 0 : i1 + 1) * s1) -
+                 ld32(vel[1], b1 + (unsigned)(i1 == 0 ? n1 - 1 : i1 - 1) * s1)) / den;
+    if (DIM == 3)
+      div = div + (ld32(vel[2], b2 + (unsigned)(i2 + 1 == n2 ? 0 : i2 + 1)) -
+                   ld32(vel[2], b2 + (unsigned)(i2 == 0 ? n2 - 1 : i2 - 1))) / den;
+    g[c] = div < (thr_scale * cs) / a.dxf ? 1 : 0;
   }
 }
 
@@ -1279,6 +1356,7 @@ struct amx_fv_ctx {
   int stepping;     // pool: a step is launched and not finished
   hipEvent_t ev_status;
   float* adj;       // reverse-mode workspace (allocated on first use)
+  unsigned char* amg;  // hybrid / AM-HLLC compression switch, one byte per cell and slot
   void* snapwork;   // snapshot diagnostics partials (allocated on first use)
   int klast;        // pool: last fixed-point launch of the step in flight
   int use_graphs;   // pool: replay captured HIP graphs of the step halves
@@ -1353,8 +1431,8 @@ amx_status validate(const amx_fv_config* c) {
   if (c->dim != 2 && c->dim != 3) return fail(AMX_ERR_UNSUPPORTED, "dimensionality must be 2 or 3%s");
   if (c->nx < 2 || c->ny < 2 || (c->dim == 3 && c->nz < 2)) return fail(AMX_ERR_INVALID, "grid too small%s");
   if (c->dim == 2 && c->nz != 1) return fail(AMX_ERR_INVALID, "nz must be 1 in 2D%s");
-  if (c->riemann < AMX_HLL || c->riemann > AMX_LAX_FRIEDRICHS)
-    return fail(AMX_ERR_UNSUPPORTED, "riemann solver must be HLL, HLLC, HLLC_LM or LAX_FRIEDRICHS%s");
+  if (c->riemann < AMX_HLL || c->riemann > AMX_AM_HLLC)
+    return fail(AMX_ERR_UNSUPPORTED, "riemann solver must be HLL, HLLC, HLLC_LM, LAX_FRIEDRICHS, HYBRID_HLLC or AM_HLLC%s");
   if (c->limiter < AMX_MINMOD || c->limiter > AMX_VAN_ALBADA)
     return fail(AMX_ERR_UNSUPPORTED, "limiter must be MINMOD, OSHER, DOUBLE_MINMOD, SUPERBEE or VAN_ALBADA%s");
   if (c->magnetic_integrator != AMX_IMPLICIT_MIDPOINT && c->magnetic_integrator != AMX_IMPLICIT_EULER)
@@ -1387,6 +1465,7 @@ void free_ctx(amx_fv_ctx* c) {
   (void)hipDeviceSynchronize();
   if (c->mem) (void)hipFree(c->mem);
   if (c->adj) (void)hipFree(c->adj);
+  if (c->amg) (void)hipFree(c->amg);
   if (c->snapwork) (void)hipFree(c->snapwork);
   if (c->ctl) (void)hipFree(c->ctl);
   if (c->dph) (void)hipFree(c->dph);
@@ -1455,6 +1534,12 @@ amx_status make_ctx(const amx_fv_config* cfg, int cap, int own_state, amx_fv_ctx
     free_ctx(c);
     return fail(AMX_ERR_NOMEM, "device workspace allocation failed%s");
   }
+  if ((cfg->riemann == AMX_HYBRID_HLLC || cfg->riemann == AMX_AM_HLLC) &&
+      hipMalloc((void**)&c->amg, (size_t)c->ncell * cap) != hipSuccess) {
+    c->amg = nullptr;
+    free_ctx(c);
+    return fail(AMX_ERR_NOMEM, "device workspace allocation failed%s");
+  }
   if (hipMalloc((void**)&c->ctl, sizeof(Ctl) * cap) != hipSuccess ||
       hipMalloc((void**)&c->dph, sizeof(Phys) * cap) != hipSuccess ||
       hipHostMalloc((void**)&c->hctl, sizeof(Ctl) * cap, hipHostMallocDefault) != hipSuccess ||
@@ -1518,6 +1603,12 @@ void launch_stage_rs(const amx_fv_ctx* c, const StageArgs& a, dim3 grid, dim3 bl
     case AMX_LAX_FRIEDRICHS:
       hipLaunchKernelGGL((k_stage<DIM, TA_DEF, SRC, HEUN, W0M, AMX_LAX_FRIEDRICHS, LIMX>), grid, block, 0, st, a);
       break;
+    case AMX_HYBRID_HLLC:
+      hipLaunchKernelGGL((k_stage<DIM, TA_DEF, SRC, HEUN, W0M, AMX_HYBRID_HLLC, LIMX>), grid, block, 0, st, a);
+      break;
+    case AMX_AM_HLLC:
+      hipLaunchKernelGGL((k_stage<DIM, TA_DEF, SRC, HEUN, W0M, AMX_AM_HLLC, LIMX>), grid, block, 0, st, a);
+      break;
     default:
       hipLaunchKernelGGL((k_stage<DIM, TA_DEF, SRC, HEUN, W0M, AMX_HLL, LIMX>), grid, block, 0, st, a);
   }
@@ -1530,6 +1621,8 @@ hipError_t launch_stage_t(const amx_fv_ctx* c, const StageArgs& a, int nspan, hi
     hipLaunchKernelGGL(k_lf_clear, dim3(nspan), dim3(NSLOT), 0, st, a.ctl, a.slot0);
     hipLaunchKernelGGL((k_lf_alpha<DIM, SRC>), dim3(LF_BLOCKS, nspan), dim3(256), 0, st, a);
   }
+  if (c->cfg.riemann == AMX_HYBRID_HLLC || c->cfg.riemann == AMX_AM_HLLC)
+    hipLaunchKernelGGL((k_am_switch<DIM, SRC>), dim3(LF_BLOCKS, nspan), dim3(256), 0, st, a);
   const dim3 grid(c->ntb * c->nta * c->ncx, nspan), block((TA_DEF + 1) * TB);
   if (c->cfg.limiter == AMX_MINMOD)
     launch_stage_rs<DIM, SRC, HEUN, W0M, false>(c, a, grid, block, st);
@@ -1549,6 +1642,8 @@ StageArgs stage_base(const amx_fv_ctx* c, float dt_scale) {
   memset(&a, 0, sizeof(a));
   a.limiter = c->cfg.limiter;
   a.dxf = (float)c->cfg.grid_spacing;
+  a.amg = c->amg;
+  a.gstride = c->own_state ? c->ncell : 0;
   a.ctl = c->ctl;
   a.phs = c->dph;
   a.sstride = c->sstride;
@@ -1829,6 +1924,7 @@ extern "C" amx_status amx_fv_workspace_bytes(const amx_fv_config* cfg, size_t* b
   if (s != AMX_OK) return s;
   if (!bytes) return fail(AMX_ERR_INVALID, "null output%s");
   *bytes = sizeof(float) * (size_t)cells(cfg) * ws_fields(cfg) + sizeof(Ctl) + sizeof(Phys);
+  if (cfg->riemann == AMX_HYBRID_HLLC || cfg->riemann == AMX_AM_HLLC) *bytes += (size_t)cells(cfg);
   return AMX_OK;
 }
 
@@ -2175,6 +2271,7 @@ extern "C" amx_status amx_fv_pool_slot_bytes(const amx_fv_config* cfg, size_t* b
   if (s != AMX_OK) return s;
   if (!bytes) return fail(AMX_ERR_INVALID, "null output%s");
   *bytes = sizeof(float) * (size_t)cells(cfg) * (ws_fields(cfg) + state_fields(cfg)) + sizeof(Ctl) + sizeof(Phys);
+  if (cfg->riemann == AMX_HYBRID_HLLC || cfg->riemann == AMX_AM_HLLC) *bytes += (size_t)cells(cfg);
   return AMX_OK;
 }
 

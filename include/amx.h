@@ -38,6 +38,8 @@ typedef enum amx_status {
 #define AMX_HLLC 1
 #define AMX_HLLC_LM 2           /* low-Mach HLLC (hll.py:206-221) */
 #define AMX_LAX_FRIEDRICHS 3    /* grid-wide alpha per direction (_lax_friedrichs.py:21-78) */
+#define AMX_HYBRID_HLLC 4       /* HLLC, HLLC-LM in compressing cells (hll.py:264-326) */
+#define AMX_AM_HLLC 5           /* the same with low-Mach dissipation control (hll.py:230-254) */
 /* Slope limiters (simulation_config.py:29-34 values; VAN_ALBADA_PP = 5 is not implemented). */
 #define AMX_MINMOD 0
 #define AMX_OSHER 1
@@ -55,7 +57,7 @@ typedef struct amx_fv_config {
   int dim;                        /* 2 or 3 (config.dimensionality) */
   int nx, ny, nz;                 /* cells per axis; nz = 1 in 2D */
   int mhd;                        /* config.mhd */
-  int riemann;                    /* AMX_HLL / AMX_HLLC / AMX_HLLC_LM / AMX_LAX_FRIEDRICHS (config.riemann_solver) */
+  int riemann;                    /* AMX_HLL ... AMX_AM_HLLC (config.riemann_solver) */
   int magnetic_integrator;        /* config.fv_magnetic_integrator */
   int max_fixed_point_iterations; /* 1000 in the reference (_magnetic_field_update.py:141) */
   float gamma;                    /* params.gamma */

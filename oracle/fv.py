@@ -81,6 +81,7 @@ def _load():
                 ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_long, P, P, P]
             getattr(lib, f"oracle_minmod_{sfx}").argtypes = [ctypes.c_long, P, P, P]
             getattr(lib, f"oracle_limit_{sfx}").argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_long, P, P, P]
+            getattr(lib, f"oracle_am_switch_{sfx}").argtypes = [ctypes.POINTER(OracleConfig), P, P]
         _lib = lib
     return _lib
 
@@ -162,7 +163,8 @@ def magnetic_update(B: np.ndarray, gas: np.ndarray, oc: OracleConfig, dt: float)
 def riemann(wl: np.ndarray, wr: np.ndarray, dim: int, axis: int, gamma: float, solver: int = 0,
             alpha: float = 0.0):
     """Interface fluxes for (ng, n) left/right primitive states; axis 1-based;
-    alpha is the Lax-Friedrichs dissipation speed (solver 3)."""
+    alpha is the Lax-Friedrichs dissipation speed (solver 3), or, for the
+    hybrid / AM HLLC solvers (4, 5), the compression switch (non-zero = set)."""
     lib = _load()
     a = np.ascontiguousarray(wl)
     b = np.ascontiguousarray(wr, dtype=a.dtype)
@@ -170,6 +172,15 @@ def riemann(wl: np.ndarray, wr: np.ndarray, dim: int, axis: int, gamma: float, s
     getattr(lib, f"oracle_riemann_{_sfx(a.dtype)}")(dim, solver, axis, float(gamma), float(alpha), a.shape[1],
                                                     _ptr(a), _ptr(b), _ptr(f))
     return f
+
+
+def am_switch(gas: np.ndarray, oc: OracleConfig) -> np.ndarray:
+    """Compression switch (0/1 per cell) of the AM / hybrid HLLC solvers."""
+    lib = _load()
+    g = np.ascontiguousarray(gas)
+    out = np.empty(g.shape[1:], dtype=np.uint8)
+    getattr(lib, f"oracle_am_switch_{_sfx(g.dtype)}")(ctypes.byref(oc), _ptr(g), _ptr(out))
+    return out
 
 
 def limit(lim: int, a: np.ndarray, b: np.ndarray, dx: float) -> np.ndarray:

@@ -15,6 +15,7 @@
 //   * _reconstruct_at_interface_unsplit_single  .../_state_evolution/reconstruction.py:397-431
 //   * _hll_solver / _hllc_solver (+ HLLC-LM)  astronomix/_finite_volume/_riemann_solver/hll.py:33-261
 //   * _lax_friedrichs_solver           astronomix/_finite_volume/_riemann_solver/_lax_friedrichs.py:21-78
+//   * _am_hllc_solver (AM / hybrid)    astronomix/_finite_volume/_riemann_solver/hll.py:264-326
 //   * _evolve_gas_state_unsplit_inner  .../_state_evolution/evolve_state.py:338-414
 //   * _evolve_gas_state_unsplit (RK2)  .../_state_evolution/evolve_state.py:418-488
 //   * magnetic_update (Pang-Wu)        astronomix/_finite_volume/_magnetic_update/_magnetic_field_update.py:36-242
@@ -43,7 +44,7 @@
 
 namespace {
 
-enum { RS_HLL = 0, RS_HLLC = 1, RS_HLLC_LM = 2, RS_LF = 3 };
+enum { RS_HLL = 0, RS_HLLC = 1, RS_HLLC_LM = 2, RS_LF = 3, RS_HYBRID = 4, RS_AM = 5 };
 enum { LIM_MINMOD = 0, LIM_OSHER = 1, LIM_DOUBLE_MINMOD = 2, LIM_SUPERBEE = 3, LIM_VAN_ALBADA = 4 };
 
 template <typename T>
@@ -217,10 +218,11 @@ inline void hll_flux(const T* wl, const T* wr, T* f, int dim, int axis, T gamma)
     f[v] = ((sp * fl[v] - sm * fr[v]) + (sm * sp) * (cr_[v] - cl_[v])) / (sp - sm);
 }
 
-// hll.py:113-261: HLLC, and with lm the HLLC-LM variant of hll.py:206-221
-// (no low-Mach dissipation control: that belongs to AM-HLLC)
+// hll.py:113-261: HLLC, with lm the HLLC-LM variant of hll.py:206-221 and
+// with lmd the low-Mach dissipation control of hll.py:230-254
 template <typename T>
-inline void hllc_flux(const T* wl, const T* wr, T* f, int dim, int axis, T gamma, bool lm = false) {
+inline void hllc_flux(const T* wl, const T* wr, T* f, int dim, int axis, T gamma, bool lm = false,
+                      bool lmd = false) {
   const int P = dim + 1;
   const int ng = dim + 2;
   const T rl = wl[0], ul = wl[axis], pl = wl[P];
@@ -263,9 +265,15 @@ inline void hllc_flux(const T* wl, const T* wr, T* f, int dim, int axis, T gamma
     SLd = SL * phi;
     SRd = SR * phi;
   }
+  T fmd = T(1);
+  if (lmd) {
+    const T ma = std::max(std::fabs(abs_velocity(wl, dim) / cl), std::fabs(abs_velocity(wr, dim) / cr));
+    fmd = std::min(T(1), ma);
+  }
   for (int v = 0; v < ng; ++v) {
     const T bulk = T(0.5) * (FL[v] + FR[v]);
-    const T diss = T(0.5) * ((SLd * (USL[v] - UL[v]) + as * (USL[v] - USR[v])) + SRd * (USR[v] - UR[v]));
+    T diss = T(0.5) * ((SLd * (USL[v] - UL[v]) + as * (USL[v] - USR[v])) + SRd * (USR[v] - UR[v]));
+    if (lmd && v >= 1 && v <= dim) diss = fmd * diss;
     T fs = bulk + diss;
     T out = SL >= 0 ? FL[v] : fs;
     out = SR <= 0 ? FR[v] : out;
@@ -299,6 +307,26 @@ T lf_alpha(const Grid<T>& g, const T* w, int axis) {
   return m;
 }
 
+// Compression switch of the AM / hybrid HLLC solvers (hll.py:281-297):
+// g = 1 where sum_d (v_d[i+1] - v_d[i-1]) / (2 dx) < -0.05 c / dx
+template <typename T>
+std::vector<unsigned char> am_switch(const Grid<T>& g, const T* w) {
+  const long nc = g.ncell;
+  const int P = g.pidx();
+  std::vector<unsigned char> out((size_t)nc);
+  const T den = T(2) * g.dx;
+  for (long c = 0; c < nc; ++c) {
+    T div = T(0);
+    for (int a = 0; a < g.dim; ++a) {
+      const T* v = w + (long)(a + 1) * nc;
+      div = div + (v[g.shift(c, a, 1)] - v[g.shift(c, a, -1)]) / den;
+    }
+    const T cs = std::sqrt((g.gamma * w[P * nc + c]) / w[c]);
+    out[c] = div < (T(-0.05) * cs) / g.dx ? 1 : 0;
+  }
+  return out;
+}
+
 // One forward-Euler unsplit stage, evolve_state.py:338-414.
 // w: gas rows (ng, ncell); out: gas rows; dt: stage step.
 template <typename T>
@@ -314,6 +342,8 @@ void gas_stage(const Grid<T>& g, const T* w, T* out, T dt) {
     for (int v = 0; v < ng; ++v) U[v * nc + c] = uc[v];
   }
   std::vector<T> F((size_t)ng * nc);
+  std::vector<unsigned char> sw;
+  if (g.riemann == RS_HYBRID || g.riemann == RS_AM) sw = am_switch(g, w);
   for (int axis = 1; axis <= g.dim; ++axis) {
     const T alpha = g.riemann == RS_LF ? lf_alpha(g, w, axis) : T(0);
     // interface fluxes F[i] at i-1/2 (reconstruction.py:412-429, hll.py)
@@ -335,6 +365,9 @@ void gas_stage(const Grid<T>& g, const T* w, T* out, T dt) {
       }
       if (g.riemann == RS_HLLC || g.riemann == RS_HLLC_LM)
         hllc_flux(wl, wr, f, g.dim, axis, g.gamma, g.riemann == RS_HLLC_LM);
+      else if (g.riemann == RS_HYBRID || g.riemann == RS_AM)
+        // the switch of the cell right of the interface (flux index i = interface i-1 | i)
+        hllc_flux(wl, wr, f, g.dim, axis, g.gamma, sw[c] != 0, g.riemann == RS_AM);
       else if (g.riemann == RS_LF)
         lf_flux(wl, wr, f, g.dim, axis, g.gamma, alpha);
       else
@@ -637,6 +670,8 @@ extern "C" {
       }                                                                                            \
       if (riemann == RS_HLLC || riemann == RS_HLLC_LM)                                             \
         hllc_flux<T>(a, b, o, dim, axis, (T)gamma, riemann == RS_HLLC_LM);                         \
+      else if (riemann == RS_HYBRID || riemann == RS_AM) /* alpha != 0: the switch is set */      \
+        hllc_flux<T>(a, b, o, dim, axis, (T)gamma, alpha != 0.0, riemann == RS_AM);                \
       else if (riemann == RS_LF)                                                                   \
         lf_flux<T>(a, b, o, dim, axis, (T)gamma, (T)alpha);                                        \
       else                                                                                         \
@@ -646,6 +681,12 @@ extern "C" {
   }                                                                                                \
   void oracle_minmod_##SFX(long n, const T* a, const T* b, T* out) {                               \
     for (long i = 0; i < n; ++i) out[i] = minmod<T>(a[i], b[i]);                                   \
+  }                                                                                                \
+  /* compression switch of the AM / hybrid HLLC solvers for a gas state */                        \
+  void oracle_am_switch_##SFX(const OracleConfig* oc, const T* w, unsigned char* out) {            \
+    Grid<T> g = make_grid<T>(oc->dim, oc->nx, oc->ny, oc->nz, oc->gamma, oc->dx, oc->riemann, oc->limiter);     \
+    const std::vector<unsigned char> s = am_switch<T>(g, w);                                       \
+    std::copy(s.begin(), s.end(), out);                                                            \
   }                                                                                                \
   /* limited gradient of n (backward, forward) gradient pairs */                                  \
   void oracle_limit_##SFX(int lim, double dx, long n, const T* a, const T* b, T* out) {            \

@@ -1,5 +1,6 @@
 """Scheme variants on the GPU through the C ABI against the CPU restatement
-(oracle/fv_oracle.cpp): the HLLC-LM and Lax-Friedrichs fluxes and the Osher,
+(oracle/fv_oracle.cpp): the HLLC-LM, hybrid / AM HLLC and Lax-Friedrichs
+fluxes and the Osher,
 double-minmod, superbee and van Albada limiters, in 2D and 3D, hydro and
 MHD. Same bar as the default scheme: equal step and fixed-point iteration
 counts, relative L1 per variable <= 1e-5 against the float32 oracle, or, for
@@ -42,6 +43,12 @@ CASES = [
      sc.MINMOD, 0.4, 0.02),
     ("mhd3d_ragged_hllclm_osher", lambda: W.random_smooth((12, 10, 72), True, seed=6), 3, True, sc.HLLC_LM,
      sc.OSHER, 0.4, 0.01),
+    ("blast3d32_am", lambda: W.blast_3d(32), 3, True, sc.AM_HLLC, sc.MINMOD, 0.3, 0.005),
+    ("blast3d24_hybrid_mc", lambda: W.blast_3d(24), 3, True, sc.HYBRID_HLLC, sc.DOUBLE_MINMOD, 0.3, 0.004),
+    ("ot64_am_superbee", lambda: W.orszag_tang_2d(64), 2, True, sc.AM_HLLC, sc.SUPERBEE, 0.4, 0.05),
+    ("kh64_hybrid", lambda: W.kelvin_helmholtz_2d(64), 2, False, sc.HYBRID_HLLC, sc.MINMOD, 0.4, 0.05),
+    ("hydro3d_ragged_am", lambda: W.random_smooth((12, 10, 72), False, seed=9, amp=0.6), 3, False, sc.AM_HLLC,
+     sc.MINMOD, 0.4, 0.01),
 ]
 
 
@@ -97,7 +104,8 @@ def test_variants_differ_from_the_default(gpu):
     params = SimulationParams(C_cfl=0.4, t_end=0.02, gamma=5.0 / 3.0)
     base, _ = gpu_run(state, W.fv_config(2, 64, mhd=True, riemann=sc.HLL), params)
     for rs, lim in ((sc.HLLC_LM, sc.MINMOD), (sc.LAX_FRIEDRICHS, sc.MINMOD), (sc.HLL, sc.SUPERBEE),
-                    (sc.HLL, sc.OSHER), (sc.HLL, sc.DOUBLE_MINMOD), (sc.HLL, sc.VAN_ALBADA)):
+                    (sc.HLL, sc.OSHER), (sc.HLL, sc.DOUBLE_MINMOD), (sc.HLL, sc.VAN_ALBADA),
+                    (sc.HYBRID_HLLC, sc.MINMOD), (sc.AM_HLLC, sc.MINMOD)):
         out, _ = gpu_run(state, W.fv_config(2, 64, mhd=True, riemann=rs)._replace(limiter=lim), params)
         assert np.abs(out - base).max() > 1e-6, (rs, lim)
 
@@ -109,3 +117,15 @@ def test_adjoint_refuses_variants(gpu):
     s = torch.from_numpy(state).cuda().requires_grad_(True)
     with pytest.raises(NotImplementedError):
         rollout(s, cfg._replace(limiter=sc.SUPERBEE), params._replace(t_end=0.01))
+
+
+def test_am_switch_fires_in_the_blast(gpu, oracle):
+    """The blast's expanding shell compresses the gas ahead of it: the
+    compression switch is set in some cells there, so the AM-HLLC run
+    differs from plain HLLC with the same dissipation control off."""
+    state = W.blast_3d(32)
+    oc = oracle.make_config(state.shape, 3, mhd=True, riemann=sc.AM_HLLC)
+    params = SimulationParams(C_cfl=0.3, t_end=0.005, gamma=5.0 / 3.0)
+    out, _ = gpu_run(state, W.fv_config(3, 32, mhd=True, riemann=sc.HLLC), params)
+    sw = oracle.am_switch(np.ascontiguousarray(out[:5]), oc)
+    assert 0 < sw.sum() < sw.size

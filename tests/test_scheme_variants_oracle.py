@@ -3,8 +3,11 @@ the Osher, double-minmod (monotonized central), superbee and van Albada
 limiters of limited_gradients.py:49-88 / limiters.py:18-64 against their
 textbook flux-limiter functions, HLLC-LM (hll.py:206-221) against Toro's
 upwind HLLC star states with the low-Mach scaling of the published method,
-and the Lax-Friedrichs flux (_lax_friedrichs.py:21-78) against its closed
-form and its grid-wide alpha."""
+the Lax-Friedrichs flux (_lax_friedrichs.py:21-78) against its closed
+form and its grid-wide alpha, and the hybrid / AM HLLC solvers
+(hll.py:264-326): their switch against the closed-form divergence of a
+sinusoidal flow, and their fluxes against HLLC / HLLC-LM with the low-Mach
+dissipation control."""
 
 import math
 
@@ -14,7 +17,7 @@ import pytest
 from tests.test_oracle import euler_flux
 
 MINMOD, OSHER, DOUBLE_MINMOD, SUPERBEE, VAN_ALBADA = 0, 1, 2, 3, 4
-HLL, HLLC, HLLC_LM, LF = 0, 1, 2, 3
+HLL, HLLC, HLLC_LM, LF, HYBRID, AM = 0, 1, 2, 3, 4, 5
 
 
 def _pairs(n=4000, seed=0):
@@ -194,3 +197,54 @@ def test_lax_friedrichs_alpha_is_grid_wide(oracle):
     ratio = ((4.0 + c[3, 3]) - a0) / ((2.0 + c[3, 3]) - a0)
     big = np.abs(d[2.0]) > 1e-3 * np.abs(d[2.0]).max()
     np.testing.assert_allclose(d[4.0][big] / d[2.0][big], ratio, rtol=1e-6)
+
+
+def test_hybrid_hllc_selects_hllc_or_hllc_lm(oracle):
+    dim, gamma = 3, 1.4
+    wl, wr = _states(300, 0.05, dim, seed=7)
+    f = oracle.riemann(wl, wr, dim, 2, gamma, HLLC)
+    flm = oracle.riemann(wl, wr, dim, 2, gamma, HLLC_LM)
+    np.testing.assert_array_equal(oracle.riemann(wl, wr, dim, 2, gamma, HYBRID, alpha=0.0), f)
+    np.testing.assert_array_equal(oracle.riemann(wl, wr, dim, 2, gamma, HYBRID, alpha=1.0), flm)
+
+
+@pytest.mark.parametrize("switch", [0.0, 1.0])
+def test_am_hllc_low_mach_dissipation_control(oracle, switch):
+    """AM-HLLC = (HLLC or HLLC-LM) with the velocity rows of the dissipation
+    scaled by min(1, max(|v_L| / c_L, |v_R| / c_R)); bulk = (F_L + F_R) / 2."""
+    dim, gamma = 3, 1.4
+    wl, wr = _states(300, 0.3, dim, seed=8)
+    base = oracle.riemann(wl, wr, dim, 1, gamma, HYBRID, alpha=switch)
+    am = oracle.riemann(wl, wr, dim, 1, gamma, AM, alpha=switch)
+    bulk = 0.5 * (euler_flux(wl, dim, 1, gamma) + euler_flux(wr, dim, 1, gamma))
+    cl, cr = np.sqrt(gamma * wl[dim + 1] / wl[0]), np.sqrt(gamma * wr[dim + 1] / wr[0])
+    vl = np.sqrt((wl[1:dim + 1] ** 2).sum(0) + 1e-20)
+    vr = np.sqrt((wr[1:dim + 1] ** 2).sum(0) + 1e-20)
+    fmd = np.minimum(1.0, np.maximum(vl / cl, vr / cr))
+    SL, SR = _toro_hllc_parts(wl, wr, dim, gamma)[:2]
+    sub = (SL < 0) & (SR > 0)
+    want = base.copy()
+    want[1:dim + 1] = bulk[1:dim + 1] + fmd * (base[1:dim + 1] - bulk[1:dim + 1])
+    np.testing.assert_allclose(am[:, sub], want[:, sub], rtol=1e-10, atol=1e-12)
+    np.testing.assert_array_equal(am[[0, dim + 1]][:, sub], base[[0, dim + 1]][:, sub])
+
+
+def test_am_switch_closed_form(oracle):
+    """v_x = -A sin(2 pi x) on a uniform gas: the central divergence is
+    -A cos(2 pi x) sin(2 pi dx) / dx, so the switch is set where
+    A cos(2 pi x) sin(2 pi dx) > 0.05 c."""
+    n, A, g = 64, 2.0, 1.4
+    dx = 1.0 / n
+    x = (np.arange(n) + 0.5) * dx
+    s = np.zeros((5, n, 8, 8))
+    s[0], s[4] = 1.0, 1.0
+    s[1] = (-A * np.sin(2 * np.pi * x))[:, None, None]
+    oc = oracle.make_config(s.shape, 3, mhd=False, riemann=AM, gamma=g, box_size=8 * dx)
+    oc.dx = dx
+    sw = oracle.am_switch(s, oc)
+    c = np.sqrt(g)
+    want = (A * np.cos(2 * np.pi * x) * np.sin(2 * np.pi * dx) > 0.05 * c)
+    margin = np.abs(A * np.cos(2 * np.pi * x) * np.sin(2 * np.pi * dx) - 0.05 * c) > 1e-9
+    assert want.any() and (~want).any()
+    np.testing.assert_array_equal(sw[:, 0, 0][margin], want[margin])
+    assert np.all(sw == sw[:, :1, :1])
