@@ -54,8 +54,9 @@ def unsupported_reason(config, registered_variables=None):
         return "finite difference mode needs periodic boundaries on every axis"
     if c.self_gravity:
         return "self-gravity is not implemented"
-    for name, flag in (("turbulent forcing", c.turbulent_forcing_config.turbulent_forcing),
-                       ("stellar wind", c.wind_config.stellar_wind),
+    if c.turbulent_forcing_config.turbulent_forcing and c.dimensionality != 3:
+        return "turbulent forcing is 3D only (a cubic box)"
+    for name, flag in (("stellar wind", c.wind_config.stellar_wind),
                        ("cosmic rays", c.cosmic_ray_config.cosmic_rays),
                        ("cooling", c.cooling_config.cooling),
                        ("neural net force", c.neural_net_force_config.neural_net_force),
@@ -186,7 +187,8 @@ def time_integration_fd(primitive_state, config, params, registered_variables, s
         print(f"Workspace size: {nbytes.value / 1024**2:.2f} MB")
     if config.print_elapsed_time:
         print("Starting simulation...")
-    stats, snaps, seconds = run_loop(ctx, state, run, config, params, registered_variables, snapshot_callable)
+    stats, snaps, seconds = run_loop(ctx, state, run, config, params, registered_variables, snapshot_callable,
+                                     kind="fd")
     if config.print_elapsed_time:
         print("Simulation finished!")
         print(f"Time elapsed: {seconds:.2f} seconds")

@@ -71,8 +71,9 @@ def unsupported_reason(config, registered_variables=None):
         return "only periodic boundaries on every axis are implemented"
     if c.self_gravity:
         return "self-gravity is not implemented"
-    for name, flag in (("turbulent forcing", c.turbulent_forcing_config.turbulent_forcing),
-                       ("stellar wind", c.wind_config.stellar_wind),
+    if c.turbulent_forcing_config.turbulent_forcing and c.dimensionality != 3:
+        return "turbulent forcing is 3D only (a cubic box)"
+    for name, flag in (("stellar wind", c.wind_config.stellar_wind),
                        ("cosmic rays", c.cosmic_ray_config.cosmic_rays),
                        ("cooling", c.cooling_config.cooling),
                        ("neural net force", c.neural_net_force_config.neural_net_force),

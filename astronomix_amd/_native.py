@@ -129,6 +129,8 @@ class AmxSnapshotOut(ctypes.Structure):
 
 
 SNAPSHOT_CALLBACK = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_float, ctypes.c_int, ctypes.c_void_p)
+STEP_HOOK = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long,
+                             ctypes.c_void_p)
 
 
 class AmxSnapshotSpec(ctypes.Structure):
@@ -187,6 +189,13 @@ SIGNATURES = {
                                                          ctypes.POINTER(AmxSnapshotSpec),
                                                          ctypes.POINTER(AmxSnapshotOut),
                                                          ctypes.POINTER(AmxFvStats), _P]),
+    "amx_fv_set_step_hook": (ctypes.c_int, [_P, _P, _P]),
+    "amx_fd_set_step_hook": (ctypes.c_int, [_P, _P, _P]),
+    "amx_forcing_spectrum": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                            ctypes.c_ulonglong, ctypes.c_ulonglong, _P]),
+    "amx_forcing_workspace_bytes": (ctypes.c_size_t, []),
+    "amx_forcing_apply": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_int, _P, _P, ctypes.c_float, ctypes.c_double, _P, _P, _P]),
     "amx_copy_f32": (ctypes.c_int, [_P, _P, ctypes.c_size_t, _P]),
     "amx_status_string": (ctypes.c_char_p, [ctypes.c_int]),
     "amx_last_error": (ctypes.c_char_p, []),

@@ -9,7 +9,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SOURCES = [os.path.join(HERE, "csrc", n) for n in ("fv_mhd.hip", "fd_mhd.hip", "snapshots.hip")]
+SOURCES = [os.path.join(HERE, "csrc", n) for n in ("fv_mhd.hip", "fd_mhd.hip", "snapshots.hip", "forcing.hip")]
 OUTPUT = os.path.join(HERE, "libamx.so")
 
 HIPCC_FLAGS = [

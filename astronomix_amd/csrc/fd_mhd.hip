@@ -875,6 +875,8 @@ struct amx_fd_ctx {
   double weno_ms;
   long weno_launches;
   void* snapwork;  // snapshot diagnostics partials (allocated on first use)
+  amx_step_hook hook;  // between-step hook of the step loop (forcing), or null
+  void* hook_user;
 };
 
 namespace {
@@ -1285,6 +1287,8 @@ amx_status fd_loop(amx_fd_ctx* c, float* state, const amx_fv_run* run, amx::snap
     if (!run->fixed_timestep) AMX_HIPCHK(fd_cfl(c, state, st));
     hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(1), 0, st, c->ctl, sp);
     AMX_HIPCHK(hipGetLastError());
+    if (c->hook && c->hook(c->hook_user, state, &c->ctl->dt, steps, st) != 0)
+      return amx_fail(AMX_ERR_INVALID, "the step hook stopped the run%s");
     amx_status s = fd_step(c, state, st);
     if (s != AMX_OK) return s;
     hipLaunchKernelGGL(k_step_end, dim3(1), dim3(1), 0, st, c->ctl);
@@ -1339,6 +1343,13 @@ extern "C" amx_status amx_fd_time_integration_snapshots(amx_fd_ctx* c, float* st
     d.work = c->snapwork;
   }
   return fd_loop(c, state, run, &d, stats, (hipStream_t)stream);
+}
+
+extern "C" amx_status amx_fd_set_step_hook(amx_fd_ctx* c, amx_step_hook hook, void* user) {
+  if (!c) return amx_fail(AMX_ERR_INVALID, "null context%s");
+  c->hook = hook;
+  c->hook_user = user;
+  return AMX_OK;
 }
 
 extern "C" amx_status amx_fd_set_timing(amx_fd_ctx* c, int enable) {

@@ -1357,6 +1357,8 @@ struct amx_fv_ctx {
   hipEvent_t ev_status;
   float* adj;       // reverse-mode workspace (allocated on first use)
   unsigned char* amg;  // hybrid / AM-HLLC compression switch, one byte per cell and slot
+  amx_step_hook hook;  // between-step hook of the step loop (forcing), or null
+  void* hook_user;
   void* snapwork;   // snapshot diagnostics partials (allocated on first use)
   int klast;        // pool: last fixed-point launch of the step in flight
   int use_graphs;   // pool: replay captured HIP graphs of the step halves
@@ -1959,6 +1961,14 @@ extern "C" amx_status amx_fv_get_timing(amx_fv_ctx* c, amx_fv_timing* out) {
   return AMX_OK;
 }
 
+extern "C" amx_status amx_fv_set_step_hook(amx_fv_ctx* c, amx_step_hook hook, void* user) {
+  if (!c) return fail(AMX_ERR_INVALID, "null context%s");
+  if (c->own_state) return fail(AMX_ERR_INVALID, "pools have no step hook%s");
+  c->hook = hook;
+  c->hook_user = user;
+  return AMX_OK;
+}
+
 extern "C" amx_status amx_fv_set_physics(amx_fv_ctx* c, float gamma, float C_cfl) {
   if (!c) return fail(AMX_ERR_INVALID, "null context%s");
   if (c->own_state) return fail(AMX_ERR_INVALID, "pool slots take gamma at amx_fv_pool_load%s");
@@ -2052,6 +2062,8 @@ amx_status fv_loop(amx_fv_ctx* c, float* state, const amx_fv_run* run, snap::Dri
     }
     hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(1), 0, st, c->ctl, sp);
     HIPCHK(hipGetLastError());
+    if (c->hook && c->hook(c->hook_user, state, &c->ctl[0].dt, steps, st) != 0)
+      return fail(AMX_ERR_INVALID, "the step hook stopped the run%s");
     int K = 0;
     amx_status s = do_step(c, state, st, !run->fixed_timestep, &K);
     if (s != AMX_OK) return s;

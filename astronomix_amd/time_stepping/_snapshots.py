@@ -8,6 +8,8 @@ The schedule runs inside the native step loop
 (``amx_fv_time_integration_snapshots`` / ``amx_fd_time_integration_snapshots``);
 each row (time point, state copy, totals, max |div B|) is written by the
 device into preallocated arrays, so the loop never copies a state to the host.
+``run_loop`` also installs the turbulent-forcing step hook when the
+configuration asks for it (time_integration.py:694-704).
 """
 
 from __future__ import annotations
@@ -63,9 +65,38 @@ def allocate(config, state):
     return arrays
 
 
-def run_loop(ctx, state, run, config, params, registered_variables, snapshot_callable=None):
+def _forcing(config, params, state):
+    """The turbulent-forcing hook of a run, or None (time_integration.py:694-704)."""
+    if not config.turbulent_forcing_config.turbulent_forcing:
+        return None
+    from astronomix_amd._physics_modules.turbulent_forcing import TurbulentForcing
+    shape = tuple(state.shape[1:])
+    if len(shape) != 3 or len(set(shape)) != 1:
+        raise NotImplementedError("turbulent forcing needs a cubic 3D grid (the reference's num_cells^3 box)")
+    return TurbulentForcing(shape, float(config.box_size), float(config.grid_spacing),
+                            float(params.turbulent_forcing_params.energy_injection_rate),
+                            rho_row=0, vel_row=1, num_vars=int(state.shape[0]), device=state.device)
+
+
+def run_loop(ctx, state, run, config, params, registered_variables, snapshot_callable=None, kind="fv"):
     """Runs the native loop of ctx on `state` in place. Returns
     (stats, SnapshotData with device tensors or None, seconds)."""
+    forcing = _forcing(config, params, state)
+    if forcing is None:
+        return _run_loop(ctx, state, run, config, params, registered_variables, snapshot_callable)
+    forcing.install(ctx, kind)
+    try:
+        out = _run_loop(ctx, state, run, config, params, registered_variables, snapshot_callable)
+    except _native.AmxError:
+        if forcing.error is not None:
+            raise forcing.error
+        raise
+    finally:
+        forcing.uninstall(ctx, kind)
+    return out
+
+
+def _run_loop(ctx, state, run, config, params, registered_variables, snapshot_callable=None):
     import torch
     recording = bool(config.return_snapshots)
     callback = None

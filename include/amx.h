@@ -339,6 +339,42 @@ amx_status amx_fd_time_integration_snapshots(amx_fd_ctx* ctx, float* state, cons
                                              const amx_snapshot_spec* spec, amx_snapshot_out* out,
                                              amx_fv_stats* stats, void* stream);
 
+/* ---------------------------------------------------------------------------
+ * Between-step hooks and turbulent forcing (time_integration.py:694-704;
+ * _physics_modules/_turbulent_forcing/_turbulent_forcing.py:14-104).
+ * ------------------------------------------------------------------------- */
+
+/* Called by the step loops once per step after dt is set on the device and
+ * before the step, on the loop's stream: `state` is the live device state,
+ * d_dt the device float holding the step's dt, `step` the step index.
+ * Work is enqueued on `stream`; non-zero stops the run with AMX_ERR_INVALID. */
+typedef int (*amx_step_hook)(void* user, float* state, const float* d_dt, long step, void* stream);
+
+/* Installs (or, with NULL, removes) the hook of a context's step loop. */
+amx_status amx_fv_set_step_hook(amx_fv_ctx* ctx, amx_step_hook hook, void* user);
+amx_status amx_fd_set_step_hook(amx_fd_ctx* ctx, amx_step_hook hook, void* user);
+
+/* Forcing spectrum of one step into cw, (3, nx, ny, nz) complex64 (interleaved
+ * float pairs): complex normal noise (this library's Philox4x64-10 stream,
+ * keyed by seed, counter = (element, step)) times sqrt(P(k)),
+ * P(k) = k^6 exp(-8 k / k_p), k_p = 4 pi / box_size, DC removed, compressive
+ * part projected out (_turbulent_forcing.py:14-75). The real forcing field is
+ * Re(ifftn(cw)) over the three spatial axes (numpy normalisation). */
+amx_status amx_forcing_spectrum(float* cw, int nx, int ny, int nz, double box_size, unsigned long long seed,
+                                unsigned long long step, void* stream);
+
+/* Bytes of device workspace amx_forcing_apply needs. */
+size_t amx_forcing_workspace_bytes(void);
+
+/* Adds amp * w to the velocity rows vel_row..vel_row+2 of `state`, amp the
+ * root of a amp^2 + b amp - Edot dt / dx^3 = 0 with a = sum rho |w|^2 / 2,
+ * b = sum rho v . w (0 if no real root or |a| <= 1e-10): the kinetic energy
+ * grows by energy_injection_rate * dt (_turbulent_forcing.py:77-104). dt is
+ * read from the device float d_dt; d_amp (optional) receives amp. */
+amx_status amx_forcing_apply(float* state, int num_vars, int nx, int ny, int nz, int rho_row, int vel_row,
+                             const float* w, const float* d_dt, float energy_injection_rate, double grid_spacing,
+                             void* work, float* d_amp, void* stream);
+
 /* Plain device copy dst[i] = src[i], i < n (16-byte vector loads/stores).
  * No counterpart in the reference: the HBM ceiling (STREAM copy) the
  * benchmark reports next to the kernels, and the known-size kernel against
