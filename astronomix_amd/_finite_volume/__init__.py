@@ -21,15 +21,21 @@ from astronomix_amd.option_classes.simulation_config import (
     HYBRID_HLLC,
     IMPLICIT_EULER,
     IMPLICIT_MIDPOINT,
+    GHOST_CELLS,
     LAX_FRIEDRICHS,
     MINMOD,
+    OPEN_BOUNDARY,
     OSHER,
+    PERIODIC_BOUNDARY,
+    REFLECTIVE_BOUNDARY,
     RK2_SSP,
     SPLIT,
     SUPERBEE,
     VAN_ALBADA,
     BoundarySettings,
 )
+import ctypes
+
 from astronomix_amd import _native
 
 
@@ -38,9 +44,10 @@ def unsupported_reason(config, registered_variables=None):
 
     The implemented path is the unsplit SSP-RK2 finite-volume scheme with the
     minmod, Osher, double-minmod, superbee or van Albada limiter, HLL, HLLC,
-    HLLC-LM, hybrid / AM HLLC or Lax-Friedrichs fluxes, periodic boundaries on every axis, 2D
-    or 3D, hydro or MHD (Pang-Wu magnetic sub-step), float32, no physics
-    modules.
+    HLLC-LM, hybrid / AM HLLC or Lax-Friedrichs fluxes, periodic, open or
+    reflective boundaries (both sides of an axis periodic or neither; the
+    hybrid / AM solvers periodic only), 2D or 3D, hydro or MHD (Pang-Wu
+    magnetic sub-step), float32, no physics modules.
     """
     c = config
     if c.solver_mode != FINITE_VOLUME:
@@ -63,12 +70,12 @@ def unsupported_reason(config, registered_variables=None):
         return "unknown fv_magnetic_integrator"
     if c.first_order_fallback:
         return "first_order_fallback is not implemented"
-    bs = c.boundary_settings
-    periodic = bs == ALL_PERIODIC or (isinstance(bs, BoundarySettings) and all(
-        (getattr(bs, ax).left_boundary, getattr(bs, ax).right_boundary) == (2, 2)
-        for ax in ("x", "y", "z")[: c.dimensionality]))
-    if not periodic:
-        return "only periodic boundaries on every axis are implemented"
+    try:
+        codes = boundary_codes(c)
+    except NotImplementedError as e:
+        return str(e)
+    if any(codes) and c.riemann_solver in (HYBRID_HLLC, AM_HLLC):
+        return "HYBRID_HLLC and AM_HLLC are implemented with periodic boundaries only"
     if c.self_gravity:
         return "self-gravity is not implemented"
     if c.turbulent_forcing_config.turbulent_forcing and c.dimensionality != 3:
@@ -89,6 +96,35 @@ def unsupported_reason(config, registered_variables=None):
     return None
 
 
+# reference boundary code (simulation_config.py:49-52) -> AMX_BC_* (amx.h)
+_BC_CODE = {PERIODIC_BOUNDARY: _native.AMX_BC_PERIODIC, OPEN_BOUNDARY: _native.AMX_BC_OPEN,
+            REFLECTIVE_BOUNDARY: _native.AMX_BC_REFLECTIVE}
+
+
+def boundary_codes(config) -> tuple:
+    """(x low, x high, y low, y high, z low, z high) AMX_BC_* codes of a
+    finite-volume configuration. Ghost cells are used only with
+    boundary_handling == GHOST_CELLS (time_integration.py:276-286,
+    evolve_state.py:350-412); otherwise every stencil rolls, i.e. periodic."""
+    bs = config.boundary_settings
+    if config.boundary_handling != GHOST_CELLS or bs is None or bs == ALL_PERIODIC:
+        return (0,) * 6
+    if not isinstance(bs, BoundarySettings):
+        raise NotImplementedError("2D / 3D runs need BoundarySettings (x, y, z)")
+    codes = []
+    for ax in ("x", "y", "z")[: config.dimensionality]:
+        side = getattr(bs, ax)
+        pair = (side.left_boundary, side.right_boundary)
+        if any(b not in _BC_CODE for b in pair):
+            raise NotImplementedError("only periodic, open and reflective boundaries are implemented "
+                                      "(not the MHD jet boundary)")
+        if (pair[0] == PERIODIC_BOUNDARY) != (pair[1] == PERIODIC_BOUNDARY):
+            raise NotImplementedError(f"axis {ax}: periodic on one side only has no defined ghost cells "
+                                      "(boundaries.py applies periodic ghosts when both sides are periodic)")
+        codes += [_BC_CODE[pair[0]], _BC_CODE[pair[1]]]
+    return tuple(codes + [0] * (6 - len(codes)))
+
+
 def require_supported(config, registered_variables=None):
     why = unsupported_reason(config, registered_variables)
     if why is not None:
@@ -105,7 +141,8 @@ def native_config(state_shape, config, gamma, C_cfl, grid_spacing=None) -> _nati
         dim=dim, nx=nx, ny=ny, nz=nz, mhd=int(bool(config.mhd)),
         riemann=int(config.riemann_solver), limiter=int(config.limiter),
         magnetic_integrator=int(config.fv_magnetic_integrator), max_fixed_point_iterations=1000,
-        gamma=float(gamma), C_cfl=float(C_cfl), fixed_point_tol=1e-5, grid_spacing=float(h))
+        gamma=float(gamma), C_cfl=float(C_cfl), fixed_point_tol=1e-5, grid_spacing=float(h),
+        boundary=(ctypes.c_int * 6)(*boundary_codes(config)))
 
 
 def _as_device_state(state):

@@ -20,6 +20,7 @@ AMX_HLLC = 1
 AMX_HLLC_LM = 2
 AMX_LAX_FRIEDRICHS = 3
 AMX_MINMOD = 0
+AMX_BC_PERIODIC, AMX_BC_OPEN, AMX_BC_REFLECTIVE = 0, 1, 2
 
 
 class AmxFvConfig(ctypes.Structure):
@@ -37,6 +38,7 @@ class AmxFvConfig(ctypes.Structure):
         ("fixed_point_tol", ctypes.c_float),
         ("limiter", ctypes.c_int),
         ("grid_spacing", ctypes.c_double),
+        ("boundary", ctypes.c_int * 6),
     ]
 
 
@@ -526,9 +528,14 @@ def copy_f32(dst, src, stream=None):
 _ctx_cache = {}
 
 
+def _field_key(cfg, f):
+    v = getattr(cfg, f)
+    return tuple(v) if isinstance(v, ctypes.Array) else v
+
+
 def _cached(kind, cls, cfg):
     import torch
-    key = (kind, torch.cuda.current_device()) + tuple(getattr(cfg, f) for f, _ in type(cfg)._fields_)
+    key = (kind, torch.cuda.current_device()) + tuple(_field_key(cfg, f) for f, _ in type(cfg)._fields_)
     ctx = _ctx_cache.get(key)
     if ctx is None:
         ctx = cls(cfg)
@@ -540,7 +547,7 @@ def fv_context(cfg: AmxFvConfig) -> FvContext:
     """Cached context per (device, grid and scheme); gamma and C_cfl are set
     on the cached context when they differ (they are run-time parameters)."""
     import torch
-    key = ("fv", torch.cuda.current_device()) + tuple(getattr(cfg, f) for f, _ in AmxFvConfig._fields_
+    key = ("fv", torch.cuda.current_device()) + tuple(_field_key(cfg, f) for f, _ in AmxFvConfig._fields_
                                                       if f not in ("gamma", "C_cfl"))
     ctx = _ctx_cache.get(key)
     if ctx is None:

@@ -22,7 +22,7 @@ from __future__ import annotations
 import numpy as np
 
 from astronomix_amd import _native
-from astronomix_amd._finite_volume import native_config, require_supported
+from astronomix_amd._finite_volume import boundary_codes, native_config, require_supported
 from astronomix_amd.option_classes.simulation_config import HLL, HLLC, MINMOD
 
 
@@ -107,6 +107,8 @@ def rollout(state, config, params, registered_variables=None, checkpoint_every: 
         raise NotImplementedError("reverse mode covers the hydro step only (the magnetic fixed point has none)")
     if config.riemann_solver not in (HLL, HLLC) or config.limiter != MINMOD:
         raise NotImplementedError("reverse mode covers the HLL / HLLC fluxes with the minmod limiter")
+    if any(boundary_codes(config)):
+        raise NotImplementedError("reverse mode covers periodic boundaries only")
     if not isinstance(state, torch.Tensor) or state.device.type != "cuda" or state.dtype != torch.float32:
         raise TypeError("state must be a float32 tensor on the GPU")
     _native.require_gpu()

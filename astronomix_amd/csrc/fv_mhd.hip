@@ -18,8 +18,11 @@
 // and each interface flux is computed once. Tiles are 64 cells wide along the
 // contiguous axis (one wavefront, coalesced) and TA rows tall; the block index
 // is swizzled so that neighbouring tiles land on the same XCD (shared L2).
-// Periodic boundaries are applied by wrapping indices (the reference's ghost
-// cells are periodic copies refreshed before every stencil, boundaries.py:296-319).
+// Boundaries are applied by mapping indices: the reference pads the state with
+// two ghost cells per side and refreshes them before every stencil
+// (_geometry/boundaries.py:74-357). Periodic ghosts are wrapped indices; open
+// and reflective ghosts (kernels instantiated with BCX) are the edge cell and
+// the mirror cell, with one velocity / field row negated at reflective walls.
 
 #include <hip/hip_runtime.h>
 
@@ -123,6 +126,41 @@ template <bool LIMX>
 __device__ __forceinline__ float slope(float wm, float w0, float wp, int lim, float dx) {
   if (LIMX) return half_slope_x(wm, w0, wp, lim, dx);
   return half_slope(wm, w0, wp);
+}
+
+// Boundary codes (AMX_BC_*) of the two sides of one axis.
+struct AxisBc {
+  int lo, hi;
+};
+
+// Cell of index j of an axis of n cells (_geometry/boundaries.py:268-357):
+// periodic wraps; open repeats the edge cell; reflective mirrors (ghost -1 is
+// cell 0, ghost -2 cell 1, ghost n cell n-1) and sets flip, the negation of
+// one row. Indices beyond the two ghost layers (rows and columns of a partial
+// tile) are clamped into the axis; no result depends on them.
+__device__ __forceinline__ int bnd_index(int j, int n, AxisBc b, bool& flip) {
+  flip = false;
+  if (j < 0) {
+    if (b.lo == AMX_BC_PERIODIC) return wrapi(j, n);
+    if (b.lo == AMX_BC_REFLECTIVE) {
+      flip = true;
+      j = -j - 1;
+    } else {
+      j = 0;
+    }
+    return j < n ? j : n - 1;
+  }
+  if (j >= n) {
+    if (b.hi == AMX_BC_PERIODIC) return wrapi(j, n);
+    if (b.hi == AMX_BC_REFLECTIVE) {
+      flip = true;
+      j = 2 * n - 1 - j;
+    } else {
+      j = n - 1;
+    }
+    return j >= 0 ? j : 0;
+  }
+  return j;
 }
 
 template <int DIM>
@@ -399,6 +437,7 @@ struct StageArgs {
   float dxf;             // float(grid_spacing): the limiters' gradient scale
   unsigned char* amg;    // hybrid / AM-HLLC: compression switch per cell of the stage input
   long gstride;          // bytes between consecutive slots' switch arrays
+  AxisBc bx, ba, bb;     // boundaries of the kernel axes x (3D), A and B (kernels with BCX)
 };
 
 // One unsplit forward-Euler stage (evolve_state.py:338-414), optionally with
@@ -410,7 +449,12 @@ struct StageArgs {
 // (TA+1)*64 A-interfaces and TA*65 B-interfaces of a plane take exactly two
 // rounds of the block. In 3D the block marches along x over lx planes with the
 // x-stencil of its own column in registers.
-template <int DIM, int TA, int SRC, bool HEUN, int W0M, int RS, bool LIMX>
+//
+// BCX: open / reflective boundaries. Ghost cells read the cell that
+// bnd_index maps them to, with the normal velocity negated at reflective
+// walls (_reflective_*_boundary, boundaries.py:323-357); the flux into the
+// ghost layer at an upper edge is computed from those values like any other.
+template <int DIM, int TA, int SRC, bool HEUN, int W0M, int RS, bool LIMX, bool BCX>
 __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(AMX_STAGE_WAVES))) void k_stage(const StageArgs a) {
   constexpr int NV = DIM + 2, P = DIM + 1, NT = (TA + 1) * TB, WW = TB + 4;
   constexpr int AXA = DIM == 3 ? 2 : 1;  // velocity row of the plane axis A
@@ -423,6 +467,7 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
   __shared__ float sFA[NV][(TA + 1) * TB];
   __shared__ float sFB[NV][TA * (TB + 1)];
   constexpr bool AMS = RS == AMX_HYBRID_HLLC || RS == AMX_AM_HLLC;
+  static_assert(!(AMS && BCX), "the compression switch of ghost cells is not defined here");
   __shared__ unsigned char sG[AMS ? (TA + 4) * WW : 1];
 
   const int slot = a.slot0 + blockIdx.y;
@@ -469,15 +514,24 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
   // 32-bit cell indices (a slot has < 2^30 cells): global loads then take a
   // scalar base and a 32-bit vector offset
   const unsigned sa = a.nb, sx = (unsigned)a.na * a.nb;
-  const int ga = wrapi(a0 + ta, a.na), gb = wrapi(b0 + tb, a.nb);
+  // plane position of the thread's column; fa / fb: reflective ghost in A / B
+  bool fa = false, fb = false;
+  const int ga = BCX ? bnd_index(a0 + ta, a.na, a.ba, fa) : wrapi(a0 + ta, a.na);
+  const int gb = BCX ? bnd_index(b0 + tb, a.nb, a.bb, fb) : wrapi(b0 + tb, a.nb);
   const unsigned col = (unsigned)ga * sa + gb;
   const bool active = owner && (a0 + ta < a.na) && (b0 + tb < a.nb);
   const float coef = ph.inv_dx * (a.dt_scale * C->dt);
+  // plane offset of x index xi (3D), and whether it is a reflective ghost
+  auto xplane = [&](int xi, bool& f) -> unsigned {
+    f = false;
+    return (unsigned)(BCX ? bnd_index(xi, a.nx, a.bx, f) : wrapi(xi, a.nx)) * sx;
+  };
 
   // this thread's halo cell (threads NT-NH .. NT-1), fixed for all planes
   const int h = tid - (NT - NH);
   int hr = 0, hc = 0;
   unsigned hoff = 0;
+  bool hfa = false, hfb = false;
   if (h >= 0) {
     if (h < 4 * TB) {
       const int q = h / TB;
@@ -489,7 +543,10 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
       const int q = j & 3;
       hc = q < 2 ? q - 2 : TB + q - 2;
     }
-    hoff = (unsigned)wrapi(a0 + hr, a.na) * sa + wrapi(b0 + hc, a.nb);
+    if (BCX)
+      hoff = (unsigned)bnd_index(a0 + hr, a.na, a.ba, hfa) * sa + bnd_index(b0 + hc, a.nb, a.bb, hfb);
+    else
+      hoff = (unsigned)wrapi(a0 + hr, a.na) * sa + wrapi(b0 + hc, a.nb);
   }
 
   int x0 = 0, x1 = 1;
@@ -501,10 +558,16 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
   if (owner) {
     if (DIM == 3) {
       float qm2[NV], qm1[NV], wl[NV], wr[NV];
-      load_w<DIM, SRC>(src, (unsigned)wrapi(x0 - 2, a.nx) * sx + col, qm2);
-      load_w<DIM, SRC>(src, (unsigned)wrapi(x0 - 1, a.nx) * sx + col, qm1);
+      bool f2, f1, fp1;
+      load_w<DIM, SRC>(src, xplane(x0 - 2, f2) + col, qm2);
+      load_w<DIM, SRC>(src, xplane(x0 - 1, f1) + col, qm1);
       load_w<DIM, SRC>(src, (unsigned)x0 * sx + col, q0);
-      load_w<DIM, SRC>(src, (unsigned)wrapi(x0 + 1, a.nx) * sx + col, q1);
+      load_w<DIM, SRC>(src, xplane(x0 + 1, fp1) + col, q1);
+      if (BCX) {
+        if (f2) qm2[1] = -qm2[1];
+        if (f1) qm1[1] = -qm1[1];
+        if (fp1) q1[1] = -q1[1];
+      }
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
         d0[v] = slope<LIMX>(qm1[v], q0[v], q1[v], lim, ldx);
@@ -521,7 +584,8 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
   // registers across the previous plane's flux phase)
   Raw<DIM, SRC> rq, rh;  // own column at x+2, halo cell at x
   Raw<DIM, W0M> r0;      // Heun centre at x
-  if (owner && DIM == 3) load_raw<DIM, SRC>(src, (unsigned)wrapi(x0 + 2, a.nx) * sx + col, rq);
+  bool fq;  // unused: the flip of a prefetched plane is recomputed when it is derived
+  if (owner && DIM == 3) load_raw<DIM, SRC>(src, xplane(x0 + 2, fq) + col, rq);
   if (h >= 0) load_raw<DIM, SRC>(src, (DIM == 3 ? (unsigned)x0 * sx : 0u) + hoff, rh);
   // the 8-field energy-fix centre is not prefetched (register pressure)
   constexpr bool PRE0 = HEUN && W0M == SRC_PLAIN;
@@ -534,10 +598,22 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
     const bool more = x + 1 < x1;
     float Fn[NV], q2[NV], hw[NV];
     Raw<DIM, W0M> r0c = r0;
-    if (owner && DIM == 3) derive_w<DIM, SRC>(rq, q2);
-    if (h >= 0) derive_w<DIM, SRC>(rh, hw);
+    if (owner && DIM == 3) {
+      derive_w<DIM, SRC>(rq, q2);
+      if (BCX) {
+        xplane(x + 2, fq);
+        if (fq) q2[1] = -q2[1];
+      }
+    }
+    if (h >= 0) {
+      derive_w<DIM, SRC>(rh, hw);
+      if (BCX) {
+        if (hfa) hw[AXA] = -hw[AXA];
+        if (hfb) hw[AXB] = -hw[AXB];
+      }
+    }
     if (more) {
-      if (owner && DIM == 3) load_raw<DIM, SRC>(src, (unsigned)wrapi(x + 3, a.nx) * sx + col, rq);
+      if (owner && DIM == 3) load_raw<DIM, SRC>(src, xplane(x + 3, fq) + col, rq);
       if (h >= 0) load_raw<DIM, SRC>(src, nplane + hoff, rh);
       if (PRE0 && active) load_raw<DIM, W0M>(w0p, nplane + col, r0);
     }
@@ -556,6 +632,11 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
       }
 #pragma unroll
       for (int v = 0; v < NV; ++v) sW[v][(ta + 2) * WW + tb + 2] = q0[v];
+      if (BCX) {
+        // a ghost column's cells enter the plane stencils with their flips
+        if (fa) sW[AXA][(ta + 2) * WW + tb + 2] = -q0[AXA];
+        if (fb) sW[AXB][(ta + 2) * WW + tb + 2] = -q0[AXB];
+      }
       if (AMS) sG[(ta + 2) * WW + tb + 2] = gsw[plane + col];
     }
     if (h >= 0) {
@@ -773,6 +854,7 @@ struct MagArgs {
   int euler;           // IMPLICIT_EULER: average = iterate
   int nx, na, nb, ntb, nta, ncx, lx;
   float inv_dx;
+  AxisBc bx, ba, bb;    // boundaries of the kernel axes x (3D), A and B (kernels with BCX)
 };
 
 // The fixed point's stopping rule on an executed pair (L, L+1) with residual
@@ -825,6 +907,29 @@ __device__ __forceinline__ void mag_derive3(const float* Bk, const float* vk, co
 
 __device__ __forceinline__ void mag_derive(const MagRaw& m, int euler, float* d) {
   mag_derive3(m.Bk, m.vk, m.B0, m.v0, euler, d);
+}
+
+// Row `row` of B and v negated: a reflective ghost of the 3-row arrays.
+__device__ __forceinline__ void mag_flip(MagRaw& m, int row) {
+  m.B0[row] = -m.B0[row];
+  m.Bk[row] = -m.Bk[row];
+  m.v0[row] = -m.v0[row];
+  m.vk[row] = -m.vk[row];
+}
+
+// (avgB, E) of the ghost copy of a cell: with row f of B and v negated, avgB_f
+// and E_k for k != f change sign (each term of E_k, k != f, has one factor of
+// row f); f < 0 is an open ghost, a plain copy.
+__device__ __forceinline__ void derived_ghost(const float* d, int f, float* o) {
+#pragma unroll
+  for (int i = 0; i < 6; ++i) o[i] = d[i];
+  if (f >= 0) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      if (k == f) o[k] = -d[k];
+      else o[3 + k] = -d[3 + k];
+    }
+  }
 }
 
 // central derivative 0.5 (f+ - f-) / dx (_vector_maths.py:58-100)
@@ -888,7 +993,17 @@ __device__ __forceinline__ float mag_resid(const float* Bk, const float* vk, con
 // neighbouring blocks that own them. Both iterates are written; the residuals
 // go to res slots L and L+1. Launch L+1's iterate is speculative: whether it
 // is the reference's result is decided by pair_final.
-template <int DIM, int TA, int TBM>
+//
+// BCX: open / reflective boundaries. The reference refreshes the ghost cells
+// of B and v before the first iterate and after each one
+// (_magnetic_field_update.py:108-110,123-125,161-163), negating row axis + 1
+// at reflective walls (By, vy at x; Bz, vz at y; none at z). Loads of ghost
+// cells take the mapped cell with that row negated; the first iterate of the
+// ghost cells next to the edge, which the second iterate reads, is the
+// refreshed copy of the edge cell's first iterate, written into the ring by
+// the edge cell's thread (the ghost's own thread, whose stencil would reach
+// past the refreshed layer, writes nothing).
+template <int DIM, int TA, int TBM, bool BCX>
 __global__ __launch_bounds__(((TA + 4) * (TBM + 4) + 255) / 256 * 256) void k_mag(const MagArgs a) {
   constexpr int WX = TBM + 4, NC = (TA + 4) * WX;   // ring-2 tile
   constexpr int W1 = TBM + 2, NC1 = (TA + 2) * W1;  // ring-1 tile
@@ -944,9 +1059,44 @@ __global__ __launch_bounds__(((TA + 4) * (TBM + 4) + 255) / 256 * 256) void k_ma
   const bool inner = has && !ring2;                       // first iterate here
   const bool tile = inner && r >= 0 && r < TA && c >= 0 && c < TBM;
   const bool own = tile && (a0 + r < a.na) && (b0 + c < a.nb);
-  const unsigned col = (unsigned)wrapi(a0 + r, a.na) * sa + wrapi(b0 + c, a.nb);
+  // rows of B and v negated at reflective walls of the kernel axes
+  constexpr int RX = 1, RA = DIM == 3 ? 2 : 1, RB = DIM == 3 ? -1 : 2;
+  bool fa = false, fb = false;
+  const unsigned col = BCX ? (unsigned)bnd_index(a0 + r, a.na, a.ba, fa) * sa + bnd_index(b0 + c, a.nb, a.bb, fb)
+                           : (unsigned)wrapi(a0 + r, a.na) * sa + wrapi(b0 + c, a.nb);
   const int o = (r + 2) * WX + (c + 2);
   const int o1 = (r + 1) * W1 + (c + 1);
+  // ghost cell of the ring-1 tile on an open / reflective side
+  const bool gh = BCX && ((a0 + r < 0 && a.ba.lo != AMX_BC_PERIODIC) || (a0 + r >= a.na && a.ba.hi != AMX_BC_PERIODIC) ||
+                          (b0 + c < 0 && a.bb.lo != AMX_BC_PERIODIC) || (b0 + c >= a.nb && a.bb.hi != AMX_BC_PERIODIC));
+  // flips of a load of the thread's column at an x plane that is (fx) or is
+  // not a reflective ghost
+  auto flips = [&](MagRaw& t, bool fx) {
+    if (BCX) {
+      if (fx) mag_flip(t, RX);
+      if (fa) mag_flip(t, RA);
+      if constexpr (RB >= 0) {
+        if (fb) mag_flip(t, RB);
+      }
+    }
+  };
+  auto xoff = [&](int xi, bool& f) -> unsigned {
+    f = false;
+    return (unsigned)(BCX ? bnd_index(xi, a.nx, a.bx, f) : wrapi(xi, a.nx)) * sx;
+  };
+  // an edge cell's first iterate e1 copied into the ghost cells beside it
+  auto ghost_copies = [&](float (*S1)[NC1], const float* e1) {
+    float g[6];
+    auto put = [&](int off, int code, int row) {
+      derived_ghost(e1, code == AMX_BC_REFLECTIVE ? row : -1, g);
+#pragma unroll
+      for (int i = 0; i < 6; ++i) S1[i][o1 + off] = g[i];
+    };
+    if (a0 + r == 0 && a.ba.lo != AMX_BC_PERIODIC) put(-W1, a.ba.lo, RA);
+    if (a0 + r == a.na - 1 && a.ba.hi != AMX_BC_PERIODIC) put(W1, a.ba.hi, RA);
+    if (b0 + c == 0 && a.bb.lo != AMX_BC_PERIODIC) put(-1, a.bb.lo, RB);
+    if (b0 + c == a.nb - 1 && a.bb.hi != AMX_BC_PERIODIC) put(1, a.bb.hi, RB);
+  };
   const float dt = C->dt;
   const float idx = a.inv_dx;
   float res0 = 0.f, res1 = 0.f;
@@ -958,6 +1108,7 @@ __global__ __launch_bounds__(((TA + 4) * (TBM + 4) + 255) / 256 * 256) void k_ma
     float d[6], rho = 1.f;
     if (has) {
       mag_load(m, col, rc);
+      flips(rc, false);
       mag_derive(rc, a.euler, d);
 #pragma unroll
       for (int i = 0; i < 6; ++i) sK[0][i][o] = d[i];
@@ -981,8 +1132,11 @@ __global__ __launch_bounds__(((TA + 4) * (TBM + 4) + 255) / 256 * 256) void k_ma
         res0 = (rr != rr || rr > res0) ? rr : res0;
       }
       mag_derive3(B1, v1, rc.B0, rc.v0, a.euler, e1);
+      if (!gh) {
 #pragma unroll
-      for (int i = 0; i < 6; ++i) sK1[0][i][o1] = e1[i];
+        for (int i = 0; i < 6; ++i) sK1[0][i][o1] = e1[i];
+      }
+      if (BCX && own) ghost_copies(sK1[0], e1);
     }
     __syncthreads();
     if (own) {
@@ -1009,15 +1163,20 @@ __global__ __launch_bounds__(((TA + 4) * (TBM + 4) + 255) / 256 * 256) void k_ma
     float rho_c = 1.f, rho_n = 1.f;
     if (has) {
       MagRaw t;
-      mag_load(m, (unsigned)wrapi(x0 - 2, a.nx) * sx + col, t);
+      bool f2, f1;
+      mag_load(m, xoff(x0 - 2, f2) + col, t);
+      flips(t, f2);
       mag_derive(t, a.euler, dm);
-      mag_load(m, (unsigned)wrapi(x0 - 1, a.nx) * sx + col, rc);
+      mag_load(m, xoff(x0 - 1, f1) + col, rc);
+      flips(rc, f1);
       mag_derive(rc, a.euler, dc);
       mag_load(m, (unsigned)x0 * sx + col, rn);
+      flips(rn, false);
       mag_derive(rn, a.euler, dp);
     }
     if (inner) {
-      rho_c = ld32(m.rho, (unsigned)wrapi(x0 - 1, a.nx) * sx + col);
+      bool unused;
+      rho_c = ld32(m.rho, xoff(x0 - 1, unused) + col);
       rho_n = ld32(m.rho, (unsigned)x0 * sx + col);
     }
     // second-iterate carries of the own column (tile threads): derived first
@@ -1032,7 +1191,8 @@ __global__ __launch_bounds__(((TA + 4) * (TBM + 4) + 255) / 256 * 256) void k_ma
       // prefetch x + 2
       MagRaw rnn;
       float rho_nn = 1.f;
-      const unsigned pnn = (unsigned)wrapi(x + 2, a.nx) * sx + col;
+      bool fnn;  // flips are applied when the prefetched plane is derived
+      const unsigned pnn = xoff(x + 2, fnn) + col;
       if (has) mag_load(m, pnn, rnn);
       if (inner) rho_nn = ld32(m.rho, pnn);
       if (has) {
@@ -1058,15 +1218,30 @@ __global__ __launch_bounds__(((TA + 4) * (TBM + 4) + 255) / 256 * 256) void k_ma
           res0 = (rr != rr || rr > res0) ? rr : res0;
         }
         mag_derive3(B1, v1, rc.B0, rc.v0, a.euler, e1);
+        if (!gh) {
 #pragma unroll
-        for (int i = 0; i < 6; ++i) sK1[bx][i][o1] = e1[i];
+          for (int i = 0; i < 6; ++i) sK1[bx][i][o1] = e1[i];
+        }
+        if (BCX && own) ghost_copies(sK1[bx], e1);
       }
       __syncthreads();
       if (own && x - 1 >= x0 && x - 1 < x1) {
         float cE[3], cB[3], B2[3], v2[3];
 #pragma unroll
         for (int i = 0; i < 6; ++i) SK1[i] = sK1[bx ^ 1][i];
-        mag_curls<3, W1>(SK1, o1, e1m, e1, idx, cE, cB);
+        if constexpr (BCX) {
+          // x neighbours past an open / reflective x edge: ghost copies of this cell
+          float xm[6], xp[6];
+          derived_ghost(e1m, -1, xm);
+          derived_ghost(e1, -1, xp);
+          if (x - 1 == 0 && a.bx.lo != AMX_BC_PERIODIC)
+            derived_ghost(e1c, a.bx.lo == AMX_BC_REFLECTIVE ? RX : -1, xm);
+          if (x - 1 == a.nx - 1 && a.bx.hi != AMX_BC_PERIODIC)
+            derived_ghost(e1c, a.bx.hi == AMX_BC_REFLECTIVE ? RX : -1, xp);
+          mag_curls<3, W1>(SK1, o1, xm, xp, idx, cE, cB);
+        } else {
+          mag_curls<3, W1>(SK1, o1, e1m, e1, idx, cE, cB);
+        }
         mag_apply(e1c, cE, cB, B0p, v0p, rhop, dt, B2, v2);
         const unsigned cell = (unsigned)(x - 1) * sx + col;
 #pragma unroll
@@ -1098,6 +1273,7 @@ __global__ __launch_bounds__(((TA + 4) * (TBM + 4) + 255) / 256 * 256) void k_ma
           dm[i] = dc[i];
           dc[i] = dp[i];
         }
+        flips(rnn, fnn);
         mag_derive(rnn, a.euler, dp);
         rc = rn;
         rn = rnn;
@@ -1341,6 +1517,8 @@ struct amx_fv_ctx {
   long ncell;
   float dx;
   int na, nb, nxm;  // kernel axes: 3D (ny, nz, nx); 2D (nx, ny, 1)
+  int bcx;          // some axis has open / reflective boundaries
+  AxisBc bx, ba, bb;  // boundaries of the kernel axes x, A, B
   int ntb, nta, ntb_mag, nta_mag, ncx, lx, ncx_mag, lx_mag;
   int cap;
   long sstride;     // floats per slot (0 for a single simulation)
@@ -1445,6 +1623,21 @@ amx_status validate(const amx_fv_config* c) {
   if (!(c->gamma > 1.0f)) return fail(AMX_ERR_INVALID, "gamma must exceed 1%s");
   if ((long)c->nx * c->ny * (c->dim == 3 ? c->nz : 1) >= (1L << 30))
     return fail(AMX_ERR_INVALID, "a grid of 2^30 cells or more is not supported (32-bit cell indices)%s");
+  int nonper = 0;
+  for (int k = 0; k < 6; ++k) {
+    const int b = c->boundary[k];
+    if (b != AMX_BC_PERIODIC && b != AMX_BC_OPEN && b != AMX_BC_REFLECTIVE)
+      return fail(AMX_ERR_INVALID, "boundary codes must be AMX_BC_PERIODIC, AMX_BC_OPEN or AMX_BC_REFLECTIVE%s");
+    if (k >= 2 * c->dim && b != AMX_BC_PERIODIC) return fail(AMX_ERR_INVALID, "2D configs have no z boundaries%s");
+    nonper |= b != AMX_BC_PERIODIC;
+  }
+  for (int a = 0; a < 3; ++a)
+    if ((c->boundary[2 * a] == AMX_BC_PERIODIC) != (c->boundary[2 * a + 1] == AMX_BC_PERIODIC))
+      return fail(AMX_ERR_UNSUPPORTED,
+                  "one side of an axis periodic and the other not has no defined ghost cells "
+                  "(boundaries.py applies periodic ghosts only when both sides are periodic)%s");
+  if (nonper && (c->riemann == AMX_HYBRID_HLLC || c->riemann == AMX_AM_HLLC))
+    return fail(AMX_ERR_UNSUPPORTED, "HYBRID_HLLC and AM_HLLC need periodic boundaries here%s");
   return AMX_OK;
 }
 
@@ -1493,15 +1686,26 @@ amx_status make_ctx(const amx_fv_config* cfg, int cap, int own_state, amx_fv_ctx
   c->NS = (int)state_fields(cfg);
   c->ncell = cells(cfg);
   c->dx = (float)cfg->grid_spacing;
+  const AxisBc bcs[3] = {{cfg->boundary[0], cfg->boundary[1]},
+                         {cfg->boundary[2], cfg->boundary[3]},
+                         {cfg->boundary[4], cfg->boundary[5]}};
   if (cfg->dim == 3) {
     c->nxm = cfg->nx;
     c->na = cfg->ny;
     c->nb = cfg->nz;
+    c->bx = bcs[0];
+    c->ba = bcs[1];
+    c->bb = bcs[2];
   } else {
     c->nxm = 1;
     c->na = cfg->nx;
     c->nb = cfg->ny;
+    c->bx = {AMX_BC_PERIODIC, AMX_BC_PERIODIC};
+    c->ba = bcs[0];
+    c->bb = bcs[1];
   }
+  c->bcx = 0;
+  for (int k = 0; k < 2 * cfg->dim; ++k) c->bcx |= cfg->boundary[k] != AMX_BC_PERIODIC;
   c->ntb = (c->nb + TB - 1) / TB;
   c->nta = (c->na + TA_DEF - 1) / TA_DEF;
   c->nta_mag = (c->na + TA_MAG - 1) / TA_MAG;
@@ -1593,27 +1797,30 @@ Bufs bufs(const amx_fv_ctx* c, float* state) {
   return b;
 }
 
-template <int DIM, int SRC, bool HEUN, int W0M, bool LIMX>
+template <int DIM, int SRC, bool HEUN, int W0M, bool LIMX, bool BCX>
 void launch_stage_rs(const amx_fv_ctx* c, const StageArgs& a, dim3 grid, dim3 block, hipStream_t st) {
+#define AMX_STAGE(RS) hipLaunchKernelGGL((k_stage<DIM, TA_DEF, SRC, HEUN, W0M, RS, LIMX, BCX>), grid, block, 0, st, a)
   switch (c->cfg.riemann) {
     case AMX_HLLC:
-      hipLaunchKernelGGL((k_stage<DIM, TA_DEF, SRC, HEUN, W0M, AMX_HLLC, LIMX>), grid, block, 0, st, a);
+      AMX_STAGE(AMX_HLLC);
       break;
     case AMX_HLLC_LM:
-      hipLaunchKernelGGL((k_stage<DIM, TA_DEF, SRC, HEUN, W0M, AMX_HLLC_LM, LIMX>), grid, block, 0, st, a);
+      AMX_STAGE(AMX_HLLC_LM);
       break;
     case AMX_LAX_FRIEDRICHS:
-      hipLaunchKernelGGL((k_stage<DIM, TA_DEF, SRC, HEUN, W0M, AMX_LAX_FRIEDRICHS, LIMX>), grid, block, 0, st, a);
+      AMX_STAGE(AMX_LAX_FRIEDRICHS);
       break;
     case AMX_HYBRID_HLLC:
-      hipLaunchKernelGGL((k_stage<DIM, TA_DEF, SRC, HEUN, W0M, AMX_HYBRID_HLLC, LIMX>), grid, block, 0, st, a);
+      // validate() rejects the switch solvers with open / reflective boundaries
+      if constexpr (!BCX) AMX_STAGE(AMX_HYBRID_HLLC);
       break;
     case AMX_AM_HLLC:
-      hipLaunchKernelGGL((k_stage<DIM, TA_DEF, SRC, HEUN, W0M, AMX_AM_HLLC, LIMX>), grid, block, 0, st, a);
+      if constexpr (!BCX) AMX_STAGE(AMX_AM_HLLC);
       break;
     default:
-      hipLaunchKernelGGL((k_stage<DIM, TA_DEF, SRC, HEUN, W0M, AMX_HLL, LIMX>), grid, block, 0, st, a);
+      AMX_STAGE(AMX_HLL);
   }
+#undef AMX_STAGE
 }
 
 template <int DIM, int SRC, bool HEUN, int W0M>
@@ -1626,10 +1833,17 @@ hipError_t launch_stage_t(const amx_fv_ctx* c, const StageArgs& a, int nspan, hi
   if (c->cfg.riemann == AMX_HYBRID_HLLC || c->cfg.riemann == AMX_AM_HLLC)
     hipLaunchKernelGGL((k_am_switch<DIM, SRC>), dim3(LF_BLOCKS, nspan), dim3(256), 0, st, a);
   const dim3 grid(c->ntb * c->nta * c->ncx, nspan), block((TA_DEF + 1) * TB);
-  if (c->cfg.limiter == AMX_MINMOD)
-    launch_stage_rs<DIM, SRC, HEUN, W0M, false>(c, a, grid, block, st);
-  else
-    launch_stage_rs<DIM, SRC, HEUN, W0M, true>(c, a, grid, block, st);
+  const bool lim = c->cfg.limiter != AMX_MINMOD;
+  if (c->bcx) {
+    if (lim)
+      launch_stage_rs<DIM, SRC, HEUN, W0M, true, true>(c, a, grid, block, st);
+    else
+      launch_stage_rs<DIM, SRC, HEUN, W0M, false, true>(c, a, grid, block, st);
+  } else if (lim) {
+    launch_stage_rs<DIM, SRC, HEUN, W0M, true, false>(c, a, grid, block, st);
+  } else {
+    launch_stage_rs<DIM, SRC, HEUN, W0M, false, false>(c, a, grid, block, st);
+  }
   return hipGetLastError();
 }
 
@@ -1658,6 +1872,9 @@ StageArgs stage_base(const amx_fv_ctx* c, float dt_scale) {
   a.ncx = c->ncx;
   a.lx = c->lx;
   a.dt_scale = dt_scale;
+  a.bx = c->bx;
+  a.ba = c->ba;
+  a.bb = c->bb;
   return a;
 }
 
@@ -1678,6 +1895,9 @@ MagArgs mag_base(const amx_fv_ctx* c) {
   m.ncx = c->ncx_mag;
   m.lx = c->lx_mag;
   m.inv_dx = (float)(1.0 / c->cfg.grid_spacing);
+  m.bx = c->bx;
+  m.ba = c->ba;
+  m.bb = c->bb;
   return m;
 }
 
@@ -1685,10 +1905,14 @@ hipError_t launch_mag(const amx_fv_ctx* c, const MagArgs& m, int nspan, hipStrea
   constexpr int NT = ((TA_MAG + 4) * (TB_MAG + 4) + 255) / 256 * 256;
   static_assert(NT <= 1024, "pair-kernel block too large");
   const dim3 grid(c->ntb_mag * c->nta_mag * c->ncx_mag, nspan), block(NT);
-  if (c->cfg.dim == 3)
-    hipLaunchKernelGGL((k_mag<3, TA_MAG, TB_MAG>), grid, block, 0, st, m);
+  if (c->cfg.dim == 3 && c->bcx)
+    hipLaunchKernelGGL((k_mag<3, TA_MAG, TB_MAG, true>), grid, block, 0, st, m);
+  else if (c->cfg.dim == 3)
+    hipLaunchKernelGGL((k_mag<3, TA_MAG, TB_MAG, false>), grid, block, 0, st, m);
+  else if (c->bcx)
+    hipLaunchKernelGGL((k_mag<2, TA_MAG, TB_MAG, true>), grid, block, 0, st, m);
   else
-    hipLaunchKernelGGL((k_mag<2, TA_MAG, TB_MAG>), grid, block, 0, st, m);
+    hipLaunchKernelGGL((k_mag<2, TA_MAG, TB_MAG, false>), grid, block, 0, st, m);
   return hipGetLastError();
 }
 
@@ -2229,6 +2453,7 @@ extern "C" amx_status amx_fv_adjoint_step(amx_fv_ctx* c, const float* state, con
   if (c->cfg.mhd) return fail(AMX_ERR_UNSUPPORTED, "reverse mode covers the hydro step only%s");
   if ((c->cfg.riemann != AMX_HLL && c->cfg.riemann != AMX_HLLC) || c->cfg.limiter != AMX_MINMOD)
     return fail(AMX_ERR_UNSUPPORTED, "reverse mode covers the HLL / HLLC fluxes with the minmod limiter%s");
+  if (c->bcx) return fail(AMX_ERR_UNSUPPORTED, "reverse mode covers periodic boundaries only%s");
   if (lam_in == lam_out || lam_in == state) return fail(AMX_ERR_INVALID, "lam_in must not alias its inputs%s");
   const int NV = c->NV, dim = c->cfg.dim;
   if (!c->adj) {

@@ -46,6 +46,17 @@ typedef enum amx_status {
 #define AMX_DOUBLE_MINMOD 2
 #define AMX_SUPERBEE 3
 #define AMX_VAN_ALBADA 4
+/* Ghost-cell boundaries of the finite-volume scheme, per side of an axis
+ * (_geometry/boundaries.py:74-357; the reference's own codes are OPEN 0,
+ * REFLECTIVE 1, PERIODIC 2, renumbered so that a zeroed config is periodic).
+ * Open repeats the edge cell; reflective mirrors the two cells next to the
+ * edge and negates the normal velocity of the gas state, and in the magnetic
+ * sub-step row axis+1 of B and v (By, vy at x walls; Bz, vz at y walls; none
+ * at z walls), as _set_specific_var_along_axis does with var index = axis.
+ * Both sides of an axis are periodic or neither is. */
+#define AMX_BC_PERIODIC 0
+#define AMX_BC_OPEN 1
+#define AMX_BC_REFLECTIVE 2
 /* Magnetic sub-step integrators (simulation_config.py:88-89 values). */
 #define AMX_IMPLICIT_MIDPOINT 0
 #define AMX_IMPLICIT_EULER 1
@@ -65,6 +76,8 @@ typedef struct amx_fv_config {
   float fixed_point_tol;          /* 1e-5 for SINGLE_PRECISION (_magnetic_field_update.py:134-139) */
   int limiter;                    /* AMX_MINMOD ... AMX_VAN_ALBADA (config.limiter) */
   double grid_spacing;            /* config.grid_spacing = box_size / num_cells */
+  int boundary[6];                /* AMX_BC_* of x low, x high, y low, y high, z low, z high
+                                     (config.boundary_settings); all zero = periodic */
 } amx_fv_config;
 
 /* Loop controls of time_integration (time_integration.py:629-801). */

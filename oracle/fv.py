@@ -39,6 +39,7 @@ class OracleConfig(ctypes.Structure):
         ("exact_end_time", ctypes.c_int),
         ("max_iter", ctypes.c_int),
         ("limiter", ctypes.c_int),
+        ("bc", ctypes.c_int * 6),
         ("gamma", ctypes.c_double),
         ("dx", ctypes.c_double),
         ("C_cfl", ctypes.c_double),
@@ -97,14 +98,17 @@ def _sfx(dtype) -> str:
 
 def make_config(state_shape, dim, *, mhd=True, riemann=0, euler=0, gamma=5.0 / 3.0,
                 box_size=1.0, C_cfl=0.4, t_end=0.2, tol=1e-5, fixed_timestep=False,
-                num_timesteps=1000, exact_end_time=True, max_iter=1000, limiter=0) -> OracleConfig:
+                num_timesteps=1000, exact_end_time=True, max_iter=1000, limiter=0,
+                bc=(0, 0, 0, 0, 0, 0)) -> OracleConfig:
+    """bc: (x low, x high, y low, y high, z low, z high), 0 periodic, 1 open, 2 reflective."""
     n = state_shape[-1]
     sh = list(state_shape[1:]) + [1]
     return OracleConfig(
         dim=dim, nx=sh[0], ny=sh[1], nz=sh[2] if dim == 3 else 1, mhd=int(mhd),
         riemann=int(riemann), euler=int(euler), fixed_timestep=int(fixed_timestep),
         num_timesteps=int(num_timesteps), exact_end_time=int(exact_end_time),
-        max_iter=int(max_iter), limiter=int(limiter), gamma=float(gamma), dx=float(box_size) / n,
+        max_iter=int(max_iter), limiter=int(limiter), bc=(ctypes.c_int * 6)(*[int(b) for b in bc]),
+        gamma=float(gamma), dx=float(box_size) / n,
         C_cfl=float(C_cfl), t_end=float(t_end), tol=float(tol))
 
 

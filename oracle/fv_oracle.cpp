@@ -24,10 +24,22 @@
 //   * _cfl_time_step (UNSPLIT branch)  astronomix/_finite_volume/_timestep_estimation/_timestep_estimator.py:94-128
 //   * step loop of _time_integration   astronomix/time_stepping/time_integration.py:629-801
 //
-// Boundaries are periodic. The reference pads the state with two ghost cells
-// and refreshes them before every stencil (boundaries.py:296-319); for periodic
-// boundaries that is the same as indexing the unpadded grid modulo N, which is
-// what this restatement does. Operation order follows the reference
+// The reference pads the state with two ghost cells and refreshes them before
+// every stencil (_geometry/boundaries.py:74-360); this restatement indexes the
+// unpadded grid and maps an index past an edge to the cell its ghost copies:
+// periodic wraps, open repeats the edge cell, reflective mirrors (ghost -1 is
+// cell 0, ghost -2 cell 1) and negates one row -- the normal velocity of the
+// gas state (var index = axis), and row `axis` of the 3-row magnetic and
+// velocity arrays of the magnetic update (the same var index, which there
+// names By/vy at x, Bz/vz at y and nothing at z: boundaries.py:324-357).
+// Pinned by tests/test_boundaries_oracle.py (mirror half of a doubled periodic
+// domain, uniform open flow, an independent padded-array restatement of the
+// magnetic iterate). Parity unpinned: the hybrid / AM-HLLC compression switch
+// with open or reflective boundaries (the reference evaluates it at ghost
+// cells from cells past the refreshed layer; this restatement takes the
+// switch of the mapped cell) -- the HIP library rejects that combination.
+//
+// Operation order follows the reference
 // expressions term by term so that fp32 runs see the same rounding sequence up
 // to the compiler's evaluation (built with -ffp-contract=off).
 //
@@ -46,6 +58,7 @@ namespace {
 
 enum { RS_HLL = 0, RS_HLLC = 1, RS_HLLC_LM = 2, RS_LF = 3, RS_HYBRID = 4, RS_AM = 5 };
 enum { LIM_MINMOD = 0, LIM_OSHER = 1, LIM_DOUBLE_MINMOD = 2, LIM_SUPERBEE = 3, LIM_VAN_ALBADA = 4 };
+enum { BC_PERIODIC = 0, BC_OPEN = 1, BC_REFLECTIVE = 2 };
 
 template <typename T>
 struct Grid {
@@ -58,6 +71,7 @@ struct Grid {
   T inv_dx;          // (1 / config.grid_spacing) rounded once from double
   int riemann;       // RS_*
   int limiter;       // LIM_*
+  int bc[3][2];      // BC_* per 0-based axis, (low, high) side
   int ng() const { return dim + 2; }        // number of gas variables
   int pidx() const { return dim + 1; }      // pressure / energy row
   long shift(long c, int axis, int s) const {
@@ -70,10 +84,33 @@ struct Grid {
     if (j >= n_) j -= n_;
     return c + (j - i) * st;
   }
+  // neighbour of cell c along axis by s (|s| <= 2) under the boundary
+  // settings; *flip tells whether the cell is a reflective ghost
+  long nbr(long c, int axis, int s, bool* flip) const {
+    const long st = stride[axis];
+    const long n_ = n[axis];
+    const long i = axis == 0 ? c / st : (c / st) % n_;
+    long j = i + s;
+    *flip = false;
+    if (j < 0) {
+      const int b = bc[axis][0];
+      if (b == BC_PERIODIC) j += n_;
+      else if (b == BC_OPEN) j = 0;
+      else { j = -j - 1; *flip = true; }
+    } else if (j >= n_) {
+      const int b = bc[axis][1];
+      if (b == BC_PERIODIC) j -= n_;
+      else if (b == BC_OPEN) j = n_ - 1;
+      else { j = 2 * n_ - 1 - j; *flip = true; }
+    }
+    return c + (j - i) * st;
+  }
+  bool periodic(int axis) const { return bc[axis][0] == BC_PERIODIC && bc[axis][1] == BC_PERIODIC; }
 };
 
 template <typename T>
-Grid<T> make_grid(int dim, int nx, int ny, int nz, double gamma, double dx, int riemann, int limiter = 0) {
+Grid<T> make_grid(int dim, int nx, int ny, int nz, double gamma, double dx, int riemann, int limiter = 0,
+                  const int* bc = nullptr) {
   Grid<T> g;
   g.dim = dim;
   g.n[0] = nx;
@@ -92,6 +129,8 @@ Grid<T> make_grid(int dim, int nx, int ny, int nz, double gamma, double dx, int 
   g.inv_dx = (T)(1.0 / dx);
   g.riemann = riemann;
   g.limiter = limiter;
+  for (int a = 0; a < 3; ++a)
+    for (int k = 0; k < 2; ++k) g.bc[a][k] = bc ? bc[2 * a + k] : BC_PERIODIC;
   return g;
 }
 
@@ -319,7 +358,9 @@ std::vector<unsigned char> am_switch(const Grid<T>& g, const T* w) {
     T div = T(0);
     for (int a = 0; a < g.dim; ++a) {
       const T* v = w + (long)(a + 1) * nc;
-      div = div + (v[g.shift(c, a, 1)] - v[g.shift(c, a, -1)]) / den;
+      bool fp, fm;
+      const long cp = g.nbr(c, a, 1, &fp), cm = g.nbr(c, a, -1, &fm);
+      div = div + ((fp ? -v[cp] : v[cp]) - (fm ? -v[cm] : v[cm])) / den;
     }
     const T cs = std::sqrt((g.gamma * w[P * nc + c]) / w[c]);
     out[c] = div < (T(-0.05) * cs) / g.dx ? 1 : 0;
@@ -346,15 +387,18 @@ void gas_stage(const Grid<T>& g, const T* w, T* out, T dt) {
   if (g.riemann == RS_HYBRID || g.riemann == RS_AM) sw = am_switch(g, w);
   for (int axis = 1; axis <= g.dim; ++axis) {
     const T alpha = g.riemann == RS_LF ? lf_alpha(g, w, axis) : T(0);
-    // interface fluxes F[i] at i-1/2 (reconstruction.py:412-429, hll.py)
-#pragma omp parallel for schedule(static)
-    for (long c = 0; c < nc; ++c) {
-      const long cm1 = g.shift(c, axis - 1, -1);
-      const long cm2 = g.shift(c, axis - 1, -2);
-      const long cp1 = g.shift(c, axis - 1, +1);
-      T wl[5], wr[5], f[5];
+    // interface flux between the cells at offsets s0 and s0 + 1 from c along
+    // axis (reconstruction.py:412-429, the Riemann solvers); cells past an
+    // edge take their ghost values (reflective ghosts negate the normal velocity)
+    auto iface = [&](long c, int s0, T* f) {
+      long id[4];
+      bool fl[4];
+      for (int q = 0; q < 4; ++q) id[q] = g.nbr(c, axis - 1, s0 - 1 + q, &fl[q]);
+      T wl[5], wr[5];
       for (int v = 0; v < ng; ++v) {
-        const T a = w[v * nc + cm2], b = w[v * nc + cm1], cc = w[v * nc + c], d = w[v * nc + cp1];
+        T x[4];
+        for (int q = 0; q < 4; ++q) x[q] = (fl[q] && v == axis) ? -w[v * nc + id[q]] : w[v * nc + id[q]];
+        const T a = x[0], b = x[1], cc = x[2], d = x[3];
         // limited_gradients.py:50-60 ; reconstruction.py:409-429
         const T g_m1 = limit(g.limiter, (b - a) / g.dx, (cc - b) / g.dx, g.dx);
         const T g_0 = limit(g.limiter, (cc - b) / g.dx, (d - cc) / g.dx, g.dx);
@@ -367,19 +411,33 @@ void gas_stage(const Grid<T>& g, const T* w, T* out, T dt) {
         hllc_flux(wl, wr, f, g.dim, axis, g.gamma, g.riemann == RS_HLLC_LM);
       else if (g.riemann == RS_HYBRID || g.riemann == RS_AM)
         // the switch of the cell right of the interface (flux index i = interface i-1 | i)
-        hllc_flux(wl, wr, f, g.dim, axis, g.gamma, sw[c] != 0, g.riemann == RS_AM);
+        hllc_flux(wl, wr, f, g.dim, axis, g.gamma, sw[id[2]] != 0, g.riemann == RS_AM);
       else if (g.riemann == RS_LF)
         lf_flux(wl, wr, f, g.dim, axis, g.gamma, alpha);
       else
         hll_flux(wl, wr, f, g.dim, axis, g.gamma);
-      for (int v = 0; v < ng; ++v) F[v * nc + c] = f[v];
-    }
-    // evolve_state.py:398-404: U += 1/dx * (F_i - F_{i+1}) * dt
+    };
 #pragma omp parallel for schedule(static)
     for (long c = 0; c < nc; ++c) {
-      const long cp1 = g.shift(c, axis - 1, +1);
+      T f[5];
+      iface(c, -1, f);
+      for (int v = 0; v < ng; ++v) F[v * nc + c] = f[v];
+    }
+    // evolve_state.py:398-404: U += 1/dx * (F_i - F_{i+1}) * dt; past a
+    // non-periodic upper edge F_{i+1} is the flux into the ghost cell
+    const bool wrap_hi = g.bc[axis - 1][1] == BC_PERIODIC;
+#pragma omp parallel for schedule(static)
+    for (long c = 0; c < nc; ++c) {
+      bool unused;
+      const long cp1 = g.nbr(c, axis - 1, +1, &unused);
+      T fb[5];
+      const long st = g.stride[axis - 1];
+      const long i = axis == 1 ? c / st : (c / st) % g.n[axis - 1];
+      const bool edge = !wrap_hi && i == g.n[axis - 1] - 1;
+      if (edge) iface(c, 0, fb);
       for (int v = 0; v < ng; ++v) {
-        const T change = (g.inv_dx * (F[v * nc + c] - F[v * nc + cp1])) * dt;
+        const T fr = edge ? fb[v] : F[v * nc + cp1];
+        const T change = (g.inv_dx * (F[v * nc + c] - fr)) * dt;
         U[v * nc + c] = U[v * nc + c] + change;
       }
     }
@@ -416,15 +474,19 @@ void gas_rk2(const Grid<T>& g, T* w, T dt) {
   }
 }
 
-// curl of a 3-component field, _vector_maths.py:58-100.
-// comp: (3, ncell). 2D drops the z derivatives (curl2D).
-template <typename T>
-inline void curl_at(const Grid<T>& g, const T* f, long nc, long c, T* out) {
+// curl of a 3-component field, _vector_maths.py:58-100, at cell c: comp(k, m,
+// fc) is component k of the field at cell m, with component fc negated when
+// fc >= 0 (a reflective ghost). 2D drops the z derivatives (curl2D).
+template <typename T, typename Get>
+inline void curl_at(const Grid<T>& g, long c, Get comp, T* out) {
   const T dx = g.dx;
-  auto d = [&](int comp, int axis) -> T {
+  auto d = [&](int k, int axis) -> T {
     // 0.5 * (roll(-1) - roll(1)) / dx along 0-based array axis
-    const long cp = g.shift(c, axis, +1), cm = g.shift(c, axis, -1);
-    return (T(0.5) * (f[comp * nc + cp] - f[comp * nc + cm])) / dx;
+    bool fp, fm;
+    const long cp = g.nbr(c, axis, +1, &fp), cm = g.nbr(c, axis, -1, &fm);
+    // reflective ghosts of the 3-row arrays negate row axis + 1 (none past 2)
+    const int fc = axis + 1 <= 2 ? axis + 1 : -1;
+    return (T(0.5) * (comp(k, cp, fp ? fc : -1) - comp(k, cm, fm ? fc : -1))) / dx;
   };
   if (g.dim == 3) {
     out[0] = d(2, 1) - d(1, 2);
@@ -464,12 +526,24 @@ T mag_apply(const Grid<T>& g, const T* avgB, const T* avgv, const T* rho, const 
     E[nc + c] = e[1];
     E[2 * nc + c] = e[2];
   }
+  // a ghost's E is the cross product of its (negated-row) ghost fields
+  auto e_at = [&](int k, long m, int fc) -> T {
+    if (fc < 0) return E[k * nc + m];
+    T b[3] = {avgB[m], avgB[nc + m], avgB[2 * nc + m]};
+    T v[3] = {avgv[m], avgv[nc + m], avgv[2 * nc + m]};
+    b[fc] = -b[fc];
+    v[fc] = -v[fc];
+    T e[3];
+    cross3(b, v, e);
+    return e[k];
+  };
+  auto b_at = [&](int k, long m, int fc) -> T { return k == fc ? -avgB[k * nc + m] : avgB[k * nc + m]; };
   T res = 0;
 #pragma omp parallel for schedule(static) reduction(max : res)
   for (long c = 0; c < nc; ++c) {
     T cE[3], cB[3], b[3] = {avgB[c], avgB[nc + c], avgB[2 * nc + c]}, p2[3];
-    curl_at(g, E.data(), nc, c, cE);
-    curl_at(g, avgB, nc, c, cB);
+    curl_at(g, c, e_at, cE);
+    curl_at(g, c, b_at, cB);
     cross3(b, cB, p2);
     T nb = 0, nv = 0;
     for (int k = 0; k < 3; ++k) {
@@ -596,13 +670,14 @@ struct OracleConfig {
   int exact_end_time;
   int max_iter;       // magnetic fixed-point cap (1000 in the reference)
   int limiter;        // 0 minmod, 1 Osher, 2 double minmod, 3 superbee, 4 van Albada
+  int bc[6];          // BC_* of x low/high, y low/high, z low/high
   double gamma, dx, C_cfl, t_end, tol;
 };
 
 // loop of _time_integration (time_integration.py:629-801) without snapshots
 template <typename T>
 long time_integration(const OracleConfig& oc, T* state, long max_steps, double* t_out, long* iters_out) {
-  Grid<T> g = make_grid<T>(oc.dim, oc.nx, oc.ny, oc.nz, oc.gamma, oc.dx, oc.riemann, oc.limiter);
+  Grid<T> g = make_grid<T>(oc.dim, oc.nx, oc.ny, oc.nz, oc.gamma, oc.dx, oc.riemann, oc.limiter, oc.bc);
   T t = 0;
   const T t_end = (T)oc.t_end;
   long steps = 0, iters = 0;
@@ -639,23 +714,23 @@ extern "C" {
     return time_integration<T>(*oc, state, max_steps, t_out, iters_out);                           \
   }                                                                                                \
   int oracle_evolve_state_fv_##SFX(const OracleConfig* oc, T* state, double dt) {                  \
-    Grid<T> g = make_grid<T>(oc->dim, oc->nx, oc->ny, oc->nz, oc->gamma, oc->dx, oc->riemann, oc->limiter);     \
+    Grid<T> g = make_grid<T>(oc->dim, oc->nx, oc->ny, oc->nz, oc->gamma, oc->dx, oc->riemann, oc->limiter, oc->bc);     \
     return evolve_state_fv<T>(g, state, oc->mhd, (T)dt, (T)oc->tol, oc->max_iter, oc->euler);      \
   }                                                                                                \
   double oracle_cfl_dt_##SFX(const OracleConfig* oc, const T* gas) {                               \
-    Grid<T> g = make_grid<T>(oc->dim, oc->nx, oc->ny, oc->nz, oc->gamma, oc->dx, oc->riemann, oc->limiter);     \
+    Grid<T> g = make_grid<T>(oc->dim, oc->nx, oc->ny, oc->nz, oc->gamma, oc->dx, oc->riemann, oc->limiter, oc->bc);     \
     return (double)cfl_dt<T>(g, gas, (T)oc->C_cfl);                                                \
   }                                                                                                \
   void oracle_gas_stage_##SFX(const OracleConfig* oc, const T* w, T* out, double dt) {             \
-    Grid<T> g = make_grid<T>(oc->dim, oc->nx, oc->ny, oc->nz, oc->gamma, oc->dx, oc->riemann, oc->limiter);     \
+    Grid<T> g = make_grid<T>(oc->dim, oc->nx, oc->ny, oc->nz, oc->gamma, oc->dx, oc->riemann, oc->limiter, oc->bc);     \
     gas_stage<T>(g, w, out, (T)dt);                                                                \
   }                                                                                                \
   void oracle_gas_rk2_##SFX(const OracleConfig* oc, T* w, double dt) {                             \
-    Grid<T> g = make_grid<T>(oc->dim, oc->nx, oc->ny, oc->nz, oc->gamma, oc->dx, oc->riemann, oc->limiter);     \
+    Grid<T> g = make_grid<T>(oc->dim, oc->nx, oc->ny, oc->nz, oc->gamma, oc->dx, oc->riemann, oc->limiter, oc->bc);     \
     gas_rk2<T>(g, w, (T)dt);                                                                       \
   }                                                                                                \
   int oracle_magnetic_update_##SFX(const OracleConfig* oc, T* B, T* gas, double dt) {              \
-    Grid<T> g = make_grid<T>(oc->dim, oc->nx, oc->ny, oc->nz, oc->gamma, oc->dx, oc->riemann, oc->limiter);     \
+    Grid<T> g = make_grid<T>(oc->dim, oc->nx, oc->ny, oc->nz, oc->gamma, oc->dx, oc->riemann, oc->limiter, oc->bc);     \
     return magnetic_update<T>(g, B, gas, (T)dt, (T)oc->tol, oc->max_iter, oc->euler);              \
   }                                                                                                \
   /* interface flux for n independent (wl, wr) pairs laid out (ng, n) */                          \
@@ -684,7 +759,7 @@ extern "C" {
   }                                                                                                \
   /* compression switch of the AM / hybrid HLLC solvers for a gas state */                        \
   void oracle_am_switch_##SFX(const OracleConfig* oc, const T* w, unsigned char* out) {            \
-    Grid<T> g = make_grid<T>(oc->dim, oc->nx, oc->ny, oc->nz, oc->gamma, oc->dx, oc->riemann, oc->limiter);     \
+    Grid<T> g = make_grid<T>(oc->dim, oc->nx, oc->ny, oc->nz, oc->gamma, oc->dx, oc->riemann, oc->limiter, oc->bc);     \
     const std::vector<unsigned char> s = am_switch<T>(g, w);                                       \
     std::copy(s.begin(), s.end(), out);                                                            \
   }                                                                                                \

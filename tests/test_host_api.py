@@ -13,7 +13,7 @@ import pytest
 
 import astronomix_amd as amx
 from astronomix_amd import _native
-from astronomix_amd._finite_volume import unsupported_reason
+from astronomix_amd._finite_volume import boundary_codes, unsupported_reason
 from astronomix_amd.option_classes import simulation_config as sc
 from astronomix_amd import workloads as W
 
@@ -80,7 +80,21 @@ def test_implemented_path_gate():
     ok = W.fv_config(3, 16)
     assert unsupported_reason(ok) is None
     assert unsupported_reason(W.fv_config(2, 16, mhd=False, riemann=sc.HLLC)) is None
-    assert "periodic" in unsupported_reason(ok._replace(boundary_settings=sc.BoundarySettings()))
+    # open / reflective ghost boundaries (BoundarySettings() is open on every side)
+    assert unsupported_reason(ok._replace(boundary_settings=sc.BoundarySettings())) is None
+    refl = sc.BoundarySettings1D(sc.REFLECTIVE_BOUNDARY, sc.REFLECTIVE_BOUNDARY)
+    per = sc.BoundarySettings1D(sc.PERIODIC_BOUNDARY, sc.PERIODIC_BOUNDARY)
+    mixed = sc.BoundarySettings(x=refl, y=per, z=sc.BoundarySettings1D())
+    assert unsupported_reason(ok._replace(boundary_settings=mixed)) is None
+    assert boundary_codes(ok._replace(boundary_settings=mixed)) == (2, 2, 0, 0, 1, 1)
+    assert boundary_codes(W.fv_config(2, 16)._replace(boundary_settings=mixed)) == (2, 2, 0, 0, 0, 0)
+    # ghost cells only with GHOST_CELLS handling; PERIODIC_ROLL rolls every stencil
+    assert boundary_codes(ok._replace(boundary_settings=mixed, boundary_handling=sc.PERIODIC_ROLL)) == (0,) * 6
+    one_sided = sc.BoundarySettings(x=sc.BoundarySettings1D(sc.PERIODIC_BOUNDARY, sc.OPEN_BOUNDARY), y=per, z=per)
+    assert "one side" in unsupported_reason(ok._replace(boundary_settings=one_sided))
+    jet = sc.BoundarySettings(x=per, y=sc.BoundarySettings1D(sc.MHD_JET_BOUNDARY, sc.OPEN_BOUNDARY), z=per)
+    assert "jet" in unsupported_reason(ok._replace(boundary_settings=jet))
+    assert "periodic" in unsupported_reason(ok._replace(boundary_settings=mixed, riemann_solver=sc.AM_HLLC))
     assert "UNSPLIT" in unsupported_reason(ok._replace(split=sc.SPLIT))
     for lim in (sc.OSHER, sc.DOUBLE_MINMOD, sc.SUPERBEE, sc.VAN_ALBADA):
         assert unsupported_reason(ok._replace(limiter=lim)) is None
