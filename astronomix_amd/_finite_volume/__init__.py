@@ -32,6 +32,7 @@ from astronomix_amd.option_classes.simulation_config import (
     SPLIT,
     SUPERBEE,
     VAN_ALBADA,
+    VAN_ALBADA_PP,
     BoundarySettings,
 )
 import ctypes
@@ -43,7 +44,8 @@ def unsupported_reason(config, registered_variables=None):
     """None when ``config`` lies on the implemented path, else why not.
 
     The implemented path is the unsplit SSP-RK2 finite-volume scheme with the
-    minmod, Osher, double-minmod, superbee or van Albada limiter, HLL, HLLC,
+    minmod, Osher, double-minmod, superbee, van Albada or positivity-preserving
+    van Albada limiter, HLL, HLLC,
     HLLC-LM, hybrid / AM HLLC or Lax-Friedrichs fluxes, periodic, open or
     reflective boundaries (both sides of an axis periodic or neither; the
     hybrid / AM solvers periodic only), 2D or 3D, hydro or MHD (Pang-Wu
@@ -62,8 +64,8 @@ def unsupported_reason(config, registered_variables=None):
         return "only the UNSPLIT scheme is implemented"
     if c.time_integrator != RK2_SSP:
         return "only the RK2_SSP integrator is implemented for the unsplit scheme"
-    if c.limiter not in (MINMOD, OSHER, DOUBLE_MINMOD, SUPERBEE, VAN_ALBADA):
-        return "the VAN_ALBADA_PP limiter is not implemented"
+    if c.limiter not in (MINMOD, OSHER, DOUBLE_MINMOD, SUPERBEE, VAN_ALBADA, VAN_ALBADA_PP):
+        return "unknown limiter"
     if c.riemann_solver not in (HLL, HLLC, HLLC_LM, LAX_FRIEDRICHS, HYBRID_HLLC, AM_HLLC):
         return "unknown Riemann solver"
     if c.fv_magnetic_integrator not in (IMPLICIT_MIDPOINT, IMPLICIT_EULER):
@@ -76,6 +78,8 @@ def unsupported_reason(config, registered_variables=None):
         return str(e)
     if any(codes) and c.riemann_solver in (HYBRID_HLLC, AM_HLLC):
         return "HYBRID_HLLC and AM_HLLC are implemented with periodic boundaries only"
+    if any(codes) and c.limiter == VAN_ALBADA_PP:
+        return "VAN_ALBADA_PP is implemented with periodic boundaries only"
     if c.self_gravity:
         return "self-gravity is not implemented"
     if c.turbulent_forcing_config.turbulent_forcing and c.dimensionality != 3:

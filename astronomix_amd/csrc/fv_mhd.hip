@@ -93,7 +93,7 @@ __device__ __forceinline__ float half_slope(float wm, float w0, float wp) {
 }
 
 // Limiter ids (simulation_config.py:29-34 values).
-enum { LIM_MINMOD = 0, LIM_OSHER = 1, LIM_DOUBLE_MINMOD = 2, LIM_SUPERBEE = 3, LIM_VAN_ALBADA = 4 };
+enum { LIM_MINMOD = 0, LIM_OSHER = 1, LIM_DOUBLE_MINMOD = 2, LIM_SUPERBEE = 3, LIM_VAN_ALBADA = 4, LIM_VAN_ALBADA_PP = 5 };
 
 // The other limiters of _calculate_limited_gradients (limited_gradients.py:49-88,
 // limiters.py:18-64) on the backward / forward gradients b, f, chosen by a
@@ -438,6 +438,12 @@ struct StageArgs {
   unsigned char* amg;    // hybrid / AM-HLLC: compression switch per cell of the stage input
   long gstride;          // bytes between consecutive slots' switch arrays
   AxisBc bx, ba, bb;     // boundaries of the kernel axes x (3D), A and B (kernels with BCX)
+  // VAN_ALBADA_PP: per-cell factors of the stage input (k_pp_factors), in
+  // state-axis order alpha_0.., kappa_0.., beta; slot s adds s * ppstride
+  float* ppf[7];
+  long ppstride;
+  float ppq;             // q = 2 / C_cfl with MHD, 1 / C_cfl without
+  int pp;                // the kernels instantiated with LIMX scale their slopes by ppf
 };
 
 // One unsplit forward-Euler stage (evolve_state.py:338-414), optionally with
@@ -469,6 +475,8 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
   constexpr bool AMS = RS == AMX_HYBRID_HLLC || RS == AMX_AM_HLLC;
   static_assert(!(AMS && BCX), "the compression switch of ghost cells is not defined here");
   __shared__ unsigned char sG[AMS ? (TA + 4) * WW : 1];
+  // VAN_ALBADA_PP factors of the plane cells: alpha_A, kappa_A, alpha_B, kappa_B, beta
+  __shared__ float sPF[LIMX ? 5 : 1][LIMX ? (TA + 4) * WW : 1];
 
   const int slot = a.slot0 + blockIdx.y;
   Ctl* const C = a.ctl + slot;
@@ -497,6 +505,23 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
   const int tid = threadIdx.x, ta = tid / TB, tb = tid % TB;
   const int lim = a.limiter;
   const float ldx = a.dxf;
+  // VAN_ALBADA_PP: slopes scaled by the factor of their cell (reconstruction.py:343-365)
+  const bool pp = LIMX && a.pp;
+  const float* pfp[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // plane factors, as sPF
+  const float* pfx[2] = {nullptr, nullptr};                             // alpha_x, kappa_x (3D)
+  if (pp) {
+    constexpr int XA = DIM == 3 ? 1 : 0, XB = DIM == 3 ? 2 : 1;  // state axes of A and B
+    const long po = slot * a.ppstride;
+    pfp[0] = a.ppf[XA] + po;
+    pfp[1] = a.ppf[DIM + XA] + po;
+    pfp[2] = a.ppf[XB] + po;
+    pfp[3] = a.ppf[DIM + XB] + po;
+    pfp[4] = a.ppf[2 * DIM] + po;
+    pfx[0] = a.ppf[0] + po;
+    pfx[1] = a.ppf[DIM] + po;
+  }
+  // factor of row v from a cell's (alpha, kappa, beta)
+  auto fsel = [&](float fa_, float fk_, float fb_, int v) { return v == 0 ? fa_ : (v == P ? fk_ : fb_); };
   const unsigned char* gsw = AMS ? a.amg + slot * a.gstride : nullptr;
   // Lax-Friedrichs: alpha per velocity row from the stage input's maxima
   float alf[3] = {0.f, 0.f, 0.f};
@@ -525,6 +550,13 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
   auto xplane = [&](int xi, bool& f) -> unsigned {
     f = false;
     return (unsigned)(BCX ? bnd_index(xi, a.nx, a.bx, f) : wrapi(xi, a.nx)) * sx;
+  };
+  // x-axis factors (3D) of the thread's column at x index xi
+  auto ppx = [&](int xi, float& fa_, float& fk_, float& fb_) {
+    const unsigned cx = (unsigned)wrapi(xi, a.nx) * sx + col;
+    fa_ = ld32(pfx[0], cx);
+    fk_ = ld32(pfx[1], cx);
+    fb_ = ld32(pfp[4], cx);
   };
 
   // this thread's halo cell (threads NT-NH .. NT-1), fixed for all planes
@@ -568,10 +600,20 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
         if (f1) qm1[1] = -qm1[1];
         if (fp1) q1[1] = -q1[1];
       }
+      float fa0 = 1.f, fk0 = 1.f, fb0 = 1.f, fam = 1.f, fkm = 1.f, fbm = 1.f;
+      if (pp) {
+        ppx(x0, fa0, fk0, fb0);
+        ppx(x0 - 1, fam, fkm, fbm);
+      }
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
         d0[v] = slope<LIMX>(qm1[v], q0[v], q1[v], lim, ldx);
-        wl[v] = qm1[v] + slope<LIMX>(qm2[v], qm1[v], q0[v], lim, ldx);
+        float sm = slope<LIMX>(qm2[v], qm1[v], q0[v], lim, ldx);
+        if (pp) {
+          d0[v] = d0[v] * fsel(fa0, fk0, fb0, v);
+          sm = sm * fsel(fam, fkm, fbm, v);
+        }
+        wl[v] = qm1[v] + sm;
         wr[v] = q0[v] - d0[v];
       }
       riemann<DIM, 1, RS>(wl, wr, Fp, ph, alf[0], AMS && gsw[(unsigned)x0 * sx + col]);
@@ -620,9 +662,12 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
     if (owner) {
       if (DIM == 3) {
         float d1[NV], wl[NV], wr[NV];
+        float fa1 = 1.f, fk1 = 1.f, fb1 = 1.f;
+        if (pp) ppx(x + 1, fa1, fk1, fb1);
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
           d1[v] = slope<LIMX>(q0[v], q1[v], q2[v], lim, ldx);
+          if (pp) d1[v] = d1[v] * fsel(fa1, fk1, fb1, v);
           wl[v] = q0[v] + d0[v];
           wr[v] = q1[v] - d1[v];
           d0[v] = d1[v];
@@ -638,11 +683,19 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
         if (fb) sW[AXB][(ta + 2) * WW + tb + 2] = -q0[AXB];
       }
       if (AMS) sG[(ta + 2) * WW + tb + 2] = gsw[plane + col];
+      if (pp) {
+#pragma unroll
+        for (int i = 0; i < 5; ++i) sPF[i][(ta + 2) * WW + tb + 2] = ld32(pfp[i], plane + col);
+      }
     }
     if (h >= 0) {
 #pragma unroll
       for (int v = 0; v < NV; ++v) sW[v][(hr + 2) * WW + hc + 2] = hw[v];
       if (AMS) sG[(hr + 2) * WW + hc + 2] = gsw[plane + hoff];
+      if (pp) {
+#pragma unroll
+        for (int i = 0; i < 5; ++i) sPF[i][(hr + 2) * WW + hc + 2] = ld32(pfp[i], plane + hoff);
+      }
     }
     __syncthreads();
     // interface fluxes of the plane; limited slopes fused in (4-point
@@ -655,8 +708,13 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
         const float w0 = sW[v][o - 2 * WW], w1 = sW[v][o - WW], w2 = sW[v][o], w3 = sW[v][o + WW];
-        wl[v] = w1 + slope<LIMX>(w0, w1, w2, lim, ldx);
-        wr[v] = w2 - slope<LIMX>(w1, w2, w3, lim, ldx);
+        float s1 = slope<LIMX>(w0, w1, w2, lim, ldx), s2 = slope<LIMX>(w1, w2, w3, lim, ldx);
+        if (pp) {
+          s1 = s1 * fsel(sPF[0][o - WW], sPF[1][o - WW], sPF[4][o - WW], v);
+          s2 = s2 * fsel(sPF[0][o], sPF[1][o], sPF[4][o], v);
+        }
+        wl[v] = w1 + s1;
+        wr[v] = w2 - s2;
       }
       riemann<DIM, AXA, RS>(wl, wr, f, ph, alf[AXA - 1], AMS && sG[o]);
 #pragma unroll
@@ -669,8 +727,13 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
         const float w0 = sW[v][o - 2], w1 = sW[v][o - 1], w2 = sW[v][o], w3 = sW[v][o + 1];
-        wl[v] = w1 + slope<LIMX>(w0, w1, w2, lim, ldx);
-        wr[v] = w2 - slope<LIMX>(w1, w2, w3, lim, ldx);
+        float s1 = slope<LIMX>(w0, w1, w2, lim, ldx), s2 = slope<LIMX>(w1, w2, w3, lim, ldx);
+        if (pp) {
+          s1 = s1 * fsel(sPF[2][o - 1], sPF[3][o - 1], sPF[4][o - 1], v);
+          s2 = s2 * fsel(sPF[2][o], sPF[3][o], sPF[4][o], v);
+        }
+        wl[v] = w1 + s1;
+        wr[v] = w2 - s2;
       }
       riemann<DIM, AXB, RS>(wl, wr, f, ph, alf[AXB - 1], AMS && sG[o]);
 #pragma unroll
@@ -834,6 +897,110 @@ __global__ __launch_bounds__(256) void k_am_switch(const StageArgs a) {
       div = div + (ld32(vel[2], b2 + (unsigned)(i2 + 1 == n2 ? 0 : i2 + 1)) -
                    ld32(vel[2], b2 + (unsigned)(i2 == 0 ? n2 - 1 : i2 - 1))) / den;
     g[c] = div < (thr_scale * cs) / a.dxf ? 1 : 0;
+  }
+}
+
+// VAN_ALBADA_PP factors of a stage input (reconstruction.py:168-365), one
+// thread per cell: the van Albada differences d[a][v] of the cell along
+// every axis, then alpha[a] = min(rho / |d[a][rho]|, 1) and kappa[a] (the
+// same for p) where |d| > 1e-14, and beta, the velocity scaling, from
+// A1 = sum_k (sum_a C[k] alpha[k] d[k][rho] d[a][v_k])^2 (the reference's
+// broadcast puts the weights on the component index), A2 = sum_a C[a]
+// sum_k d[a][v_k]^2 and q: beta = min(sqrt((q-2)^2 rho p / ((gamma-1)
+// (2 A1 + (q-2) rho^2 A2))), 1) where sum_a sum_k d[a][v_k]^2 > 1e-14.
+// C = alpha_lax / sum(alpha_lax) from the Lax-Friedrichs maxima of the same
+// input (k_lf_alpha). (1 + 1e-14) is 1 in float32. Periodic grids.
+template <int DIM, int SRC>
+__global__ __launch_bounds__(256) void k_pp_factors(const StageArgs a) {
+  constexpr int NV = DIM + 2, P = DIM + 1;
+  const int slot = a.slot0 + blockIdx.y;
+  Ctl* const C = a.ctl + slot;
+  if (!C->run) return;  // block-uniform
+  const long so = slot * a.sstride;
+  const int par = C->iters % 3;
+  const float* src[8];
+#pragma unroll
+  for (int v = 0; v < 8; ++v) src[v] = a.src[v] ? a.src[v] + so : nullptr;
+  if (SRC == SRC_MAGFIX) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) src[5 + k] = a.selv[par][k] + so;
+  }
+  float* out[2 * DIM + 1];
+#pragma unroll
+  for (int i = 0; i < 2 * DIM + 1; ++i) out[i] = a.ppf[i] + slot * a.ppstride;
+  const Phys ph = a.phs[slot];
+  float Cw[3];
+  {
+    float al[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) al[k] = __uint_as_float(wave_max_u(C->lfp[k][threadIdx.x & 63]));
+    float sum = al[0] + al[1];
+    if (DIM == 3) sum = sum + al[2];
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) Cw[k] = al[k] / sum;
+  }
+  const float qm2 = a.ppq - 2.f;
+  const float eps = 1e-14f, dx = a.dxf;
+  // axes of the state: 3D (nx, na, nb) = (x, y, z); 2D (na, nb) = (x, y)
+  const int n0 = DIM == 3 ? a.nx : a.na, n1 = DIM == 3 ? a.na : a.nb, n2 = DIM == 3 ? a.nb : 1;
+  const unsigned n = (unsigned)n0 * (unsigned)n1 * (unsigned)n2;
+  const unsigned s0 = (unsigned)n1 * (unsigned)n2, s1 = (unsigned)n2;
+  for (unsigned c = blockIdx.x * 256u + threadIdx.x; c < n; c += gridDim.x * 256u) {
+    const int i2 = (int)(c % (unsigned)n2);
+    const unsigned r = c / (unsigned)n2;
+    const int i1 = (int)(r % (unsigned)n1);
+    const int i0 = (int)(r / (unsigned)n1);
+    const int ii[3] = {i0, i1, i2};
+    const int nn[3] = {n0, n1, n2};
+    const unsigned ss[3] = {s0, s1, 1u};
+    float w[NV];
+    load_w<DIM, SRC>(src, c, w);
+    float d[DIM][NV];
+#pragma unroll
+    for (int ax = 0; ax < DIM; ++ax) {
+      const unsigned base = c - (unsigned)ii[ax] * ss[ax];
+      float wm[NV], wp[NV];
+      load_w<DIM, SRC>(src, base + (unsigned)(ii[ax] == 0 ? nn[ax] - 1 : ii[ax] - 1) * ss[ax], wm);
+      load_w<DIM, SRC>(src, base + (unsigned)(ii[ax] + 1 == nn[ax] ? 0 : ii[ax] + 1) * ss[ax], wp);
+#pragma unroll
+      for (int v = 0; v < NV; ++v) d[ax][v] = half_slope_x(wm[v], w[v], wp[v], LIM_VAN_ALBADA, dx);
+    }
+    const float rho = w[0], p = w[P];
+    float al[DIM], ka[DIM];
+#pragma unroll
+    for (int ax = 0; ax < DIM; ++ax) {
+      al[ax] = fabsf(d[ax][0]) > eps ? fminf(rho / fabsf(d[ax][0]), 1.f) : 1.f;
+      ka[ax] = fabsf(d[ax][P]) > eps ? fminf(p / fabsf(d[ax][P]), 1.f) : 1.f;
+    }
+    float A1 = 0.f, A2 = 0.f, vsum = 0.f;
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) {
+      const float wk = (Cw[k] * al[k]) * d[k][0];
+      float sk = wk * d[0][1 + k];
+#pragma unroll
+      for (int ax = 1; ax < DIM; ++ax) sk = sk + wk * d[ax][1 + k];
+      A1 = k == 0 ? sk * sk : A1 + sk * sk;
+    }
+#pragma unroll
+    for (int ax = 0; ax < DIM; ++ax) {
+      float vs = d[ax][1] * d[ax][1];
+#pragma unroll
+      for (int k = 1; k < DIM; ++k) vs = vs + d[ax][1 + k] * d[ax][1 + k];
+      A2 = ax == 0 ? Cw[ax] * vs : A2 + Cw[ax] * vs;
+      vsum = ax == 0 ? vs : vsum + vs;
+    }
+    float beta = 1.f;
+    if (vsum > eps) {
+      const float num = ((qm2 * qm2) * rho) * p;
+      const float den = (ph.gamma - 1.f) * (2.f * A1 + (qm2 * (rho * rho)) * A2);
+      beta = fminf(sqrtf(num / den), 1.f);
+    }
+#pragma unroll
+    for (int ax = 0; ax < DIM; ++ax) {
+      st32(out[ax], c, al[ax]);
+      st32(out[DIM + ax], c, ka[ax]);
+    }
+    st32(out[2 * DIM], c, beta);
   }
 }
 
@@ -1535,6 +1702,7 @@ struct amx_fv_ctx {
   hipEvent_t ev_status;
   float* adj;       // reverse-mode workspace (allocated on first use)
   unsigned char* amg;  // hybrid / AM-HLLC compression switch, one byte per cell and slot
+  float* ppf;       // VAN_ALBADA_PP factors: 2 dim + 1 fields per cell and slot
   amx_step_hook hook;  // between-step hook of the step loop (forcing), or null
   void* hook_user;
   void* snapwork;   // snapshot diagnostics partials (allocated on first use)
@@ -1613,8 +1781,11 @@ amx_status validate(const amx_fv_config* c) {
   if (c->dim == 2 && c->nz != 1) return fail(AMX_ERR_INVALID, "nz must be 1 in 2D%s");
   if (c->riemann < AMX_HLL || c->riemann > AMX_AM_HLLC)
     return fail(AMX_ERR_UNSUPPORTED, "riemann solver must be HLL, HLLC, HLLC_LM, LAX_FRIEDRICHS, HYBRID_HLLC or AM_HLLC%s");
-  if (c->limiter < AMX_MINMOD || c->limiter > AMX_VAN_ALBADA)
-    return fail(AMX_ERR_UNSUPPORTED, "limiter must be MINMOD, OSHER, DOUBLE_MINMOD, SUPERBEE or VAN_ALBADA%s");
+  if (c->limiter < AMX_MINMOD || c->limiter > AMX_VAN_ALBADA_PP)
+    return fail(AMX_ERR_UNSUPPORTED,
+                "limiter must be MINMOD, OSHER, DOUBLE_MINMOD, SUPERBEE, VAN_ALBADA or VAN_ALBADA_PP%s");
+  if (c->limiter == AMX_VAN_ALBADA_PP && !(c->C_cfl > 0.0f))
+    return fail(AMX_ERR_INVALID, "VAN_ALBADA_PP needs a positive C_cfl (its q = 1 / C_cfl)%s");
   if (c->magnetic_integrator != AMX_IMPLICIT_MIDPOINT && c->magnetic_integrator != AMX_IMPLICIT_EULER)
     return fail(AMX_ERR_UNSUPPORTED, "unknown magnetic integrator%s");
   if (c->max_fixed_point_iterations < 0 || c->max_fixed_point_iterations > 100000)
@@ -1638,6 +1809,8 @@ amx_status validate(const amx_fv_config* c) {
                   "(boundaries.py applies periodic ghosts only when both sides are periodic)%s");
   if (nonper && (c->riemann == AMX_HYBRID_HLLC || c->riemann == AMX_AM_HLLC))
     return fail(AMX_ERR_UNSUPPORTED, "HYBRID_HLLC and AM_HLLC need periodic boundaries here%s");
+  if (nonper && c->limiter == AMX_VAN_ALBADA_PP)
+    return fail(AMX_ERR_UNSUPPORTED, "VAN_ALBADA_PP needs periodic boundaries here%s");
   return AMX_OK;
 }
 
@@ -1661,6 +1834,7 @@ void free_ctx(amx_fv_ctx* c) {
   if (c->mem) (void)hipFree(c->mem);
   if (c->adj) (void)hipFree(c->adj);
   if (c->amg) (void)hipFree(c->amg);
+  if (c->ppf) (void)hipFree(c->ppf);
   if (c->snapwork) (void)hipFree(c->snapwork);
   if (c->ctl) (void)hipFree(c->ctl);
   if (c->dph) (void)hipFree(c->dph);
@@ -1746,6 +1920,12 @@ amx_status make_ctx(const amx_fv_config* cfg, int cap, int own_state, amx_fv_ctx
     free_ctx(c);
     return fail(AMX_ERR_NOMEM, "device workspace allocation failed%s");
   }
+  if (cfg->limiter == AMX_VAN_ALBADA_PP &&
+      hipMalloc((void**)&c->ppf, sizeof(float) * (size_t)(2 * cfg->dim + 1) * c->ncell * cap) != hipSuccess) {
+    c->ppf = nullptr;
+    free_ctx(c);
+    return fail(AMX_ERR_NOMEM, "device workspace allocation failed%s");
+  }
   if (hipMalloc((void**)&c->ctl, sizeof(Ctl) * cap) != hipSuccess ||
       hipMalloc((void**)&c->dph, sizeof(Phys) * cap) != hipSuccess ||
       hipHostMalloc((void**)&c->hctl, sizeof(Ctl) * cap, hipHostMallocDefault) != hipSuccess ||
@@ -1825,11 +2005,13 @@ void launch_stage_rs(const amx_fv_ctx* c, const StageArgs& a, dim3 grid, dim3 bl
 
 template <int DIM, int SRC, bool HEUN, int W0M>
 hipError_t launch_stage_t(const amx_fv_ctx* c, const StageArgs& a, int nspan, hipStream_t st) {
-  if (c->cfg.riemann == AMX_LAX_FRIEDRICHS) {
+  const bool pp = c->cfg.limiter == AMX_VAN_ALBADA_PP;
+  if (c->cfg.riemann == AMX_LAX_FRIEDRICHS || pp) {
     // the grid-wide alpha of every slot's stage input, reduced before the stage
     hipLaunchKernelGGL(k_lf_clear, dim3(nspan), dim3(NSLOT), 0, st, a.ctl, a.slot0);
     hipLaunchKernelGGL((k_lf_alpha<DIM, SRC>), dim3(LF_BLOCKS, nspan), dim3(256), 0, st, a);
   }
+  if (pp) hipLaunchKernelGGL((k_pp_factors<DIM, SRC>), dim3(LF_BLOCKS, nspan), dim3(256), 0, st, a);
   if (c->cfg.riemann == AMX_HYBRID_HLLC || c->cfg.riemann == AMX_AM_HLLC)
     hipLaunchKernelGGL((k_am_switch<DIM, SRC>), dim3(LF_BLOCKS, nspan), dim3(256), 0, st, a);
   const dim3 grid(c->ntb * c->nta * c->ncx, nspan), block((TA_DEF + 1) * TB);
@@ -1856,7 +2038,14 @@ hipError_t launch_stage(const amx_fv_ctx* c, const StageArgs& a, int nspan, hipS
 StageArgs stage_base(const amx_fv_ctx* c, float dt_scale) {
   StageArgs a;
   memset(&a, 0, sizeof(a));
-  a.limiter = c->cfg.limiter;
+  a.pp = c->cfg.limiter == AMX_VAN_ALBADA_PP;
+  a.limiter = a.pp ? (int)LIM_VAN_ALBADA : c->cfg.limiter;  // PP: van Albada slopes, then the factors
+  a.ppq = (c->cfg.mhd ? 2.0f : 1.0f) / c->cfg.C_cfl;
+  if (a.pp) {
+    const long n = c->ncell;
+    for (int i = 0; i < 2 * c->cfg.dim + 1; ++i) a.ppf[i] = c->ppf + (long)i * n;
+    a.ppstride = (long)(2 * c->cfg.dim + 1) * n;
+  }
   a.dxf = (float)c->cfg.grid_spacing;
   a.amg = c->amg;
   a.gstride = c->own_state ? c->ncell : 0;
@@ -2151,6 +2340,7 @@ extern "C" amx_status amx_fv_workspace_bytes(const amx_fv_config* cfg, size_t* b
   if (!bytes) return fail(AMX_ERR_INVALID, "null output%s");
   *bytes = sizeof(float) * (size_t)cells(cfg) * ws_fields(cfg) + sizeof(Ctl) + sizeof(Phys);
   if (cfg->riemann == AMX_HYBRID_HLLC || cfg->riemann == AMX_AM_HLLC) *bytes += (size_t)cells(cfg);
+  if (cfg->limiter == AMX_VAN_ALBADA_PP) *bytes += sizeof(float) * (size_t)cells(cfg) * (2 * cfg->dim + 1);
   return AMX_OK;
 }
 
@@ -2509,6 +2699,7 @@ extern "C" amx_status amx_fv_pool_slot_bytes(const amx_fv_config* cfg, size_t* b
   if (!bytes) return fail(AMX_ERR_INVALID, "null output%s");
   *bytes = sizeof(float) * (size_t)cells(cfg) * (ws_fields(cfg) + state_fields(cfg)) + sizeof(Ctl) + sizeof(Phys);
   if (cfg->riemann == AMX_HYBRID_HLLC || cfg->riemann == AMX_AM_HLLC) *bytes += (size_t)cells(cfg);
+  if (cfg->limiter == AMX_VAN_ALBADA_PP) *bytes += sizeof(float) * (size_t)cells(cfg) * (2 * cfg->dim + 1);
   return AMX_OK;
 }
 

@@ -11,7 +11,8 @@
  * (num_vars, nx, ny[, nz]), row-major, with
  *   2D hydro [rho, vx, vy, p]            2D MHD [rho, vx, vy, p, Bx, By, Bz]
  *   3D hydro [rho, vx, vy, vz, p]        3D MHD [rho, vx, vy, vz, p, Bx, By, Bz]
- * Boundaries are periodic on every axis.
+ * Finite-volume boundaries are periodic, open or reflective per side
+ * (amx_fv_config.boundary); the finite-difference path is periodic.
  *
  * The reference has no FFI layer of its own; each entry point below replaces
  * one JAX operator of the reference's finite-volume path, cited per function.
@@ -40,12 +41,13 @@ typedef enum amx_status {
 #define AMX_LAX_FRIEDRICHS 3    /* grid-wide alpha per direction (_lax_friedrichs.py:21-78) */
 #define AMX_HYBRID_HLLC 4       /* HLLC, HLLC-LM in compressing cells (hll.py:264-326) */
 #define AMX_AM_HLLC 5           /* the same with low-Mach dissipation control (hll.py:230-254) */
-/* Slope limiters (simulation_config.py:29-34 values; VAN_ALBADA_PP = 5 is not implemented). */
+/* Slope limiters (simulation_config.py:29-34 values). */
 #define AMX_MINMOD 0
 #define AMX_OSHER 1
 #define AMX_DOUBLE_MINMOD 2
 #define AMX_SUPERBEE 3
 #define AMX_VAN_ALBADA 4
+#define AMX_VAN_ALBADA_PP 5     /* van Albada with the positivity scaling of reconstruction.py:168-365 */
 /* Ghost-cell boundaries of the finite-volume scheme, per side of an axis
  * (_geometry/boundaries.py:74-357; the reference's own codes are OPEN 0,
  * REFLECTIVE 1, PERIODIC 2, renumbered so that a zeroed config is periodic).
@@ -74,7 +76,7 @@ typedef struct amx_fv_config {
   float gamma;                    /* params.gamma */
   float C_cfl;                    /* params.C_cfl */
   float fixed_point_tol;          /* 1e-5 for SINGLE_PRECISION (_magnetic_field_update.py:134-139) */
-  int limiter;                    /* AMX_MINMOD ... AMX_VAN_ALBADA (config.limiter) */
+  int limiter;                    /* AMX_MINMOD ... AMX_VAN_ALBADA_PP (config.limiter) */
   double grid_spacing;            /* config.grid_spacing = box_size / num_cells */
   int boundary[6];                /* AMX_BC_* of x low, x high, y low, y high, z low, z high
                                      (config.boundary_settings); all zero = periodic */

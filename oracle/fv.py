@@ -83,6 +83,7 @@ def _load():
             getattr(lib, f"oracle_minmod_{sfx}").argtypes = [ctypes.c_long, P, P, P]
             getattr(lib, f"oracle_limit_{sfx}").argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_long, P, P, P]
             getattr(lib, f"oracle_am_switch_{sfx}").argtypes = [ctypes.POINTER(OracleConfig), P, P]
+            getattr(lib, f"oracle_pp_factors_{sfx}").argtypes = [ctypes.POINTER(OracleConfig), P, P]
         _lib = lib
     return _lib
 
@@ -184,6 +185,16 @@ def am_switch(gas: np.ndarray, oc: OracleConfig) -> np.ndarray:
     g = np.ascontiguousarray(gas)
     out = np.empty(g.shape[1:], dtype=np.uint8)
     getattr(lib, f"oracle_am_switch_{_sfx(g.dtype)}")(ctypes.byref(oc), _ptr(g), _ptr(out))
+    return out
+
+
+def pp_factors(gas: np.ndarray, oc: OracleConfig) -> np.ndarray:
+    """VAN_ALBADA_PP factors of a gas state, (2 dim + 1,) + grid: alpha per
+    axis, kappa per axis, beta (reconstruction.py:168-365)."""
+    lib = _load()
+    g = np.ascontiguousarray(gas)
+    out = np.empty((2 * oc.dim + 1,) + g.shape[1:], dtype=g.dtype)
+    getattr(lib, f"oracle_pp_factors_{_sfx(g.dtype)}")(ctypes.byref(oc), _ptr(g), _ptr(out))
     return out
 
 

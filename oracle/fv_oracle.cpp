@@ -13,6 +13,8 @@
 //                                      astronomix/_finite_volume/_state_evolution/limiters.py:13-64
 //   * _calculate_limited_gradients     .../_state_evolution/limited_gradients.py:21-88 (CARTESIAN)
 //   * _reconstruct_at_interface_unsplit_single  .../_state_evolution/reconstruction.py:397-431
+//   * VAN_ALBADA_PP scaling of _reconstruct_at_interface_unsplit  .../reconstruction.py:168-365
+//       (pp_factors; pinned by tests/test_pp_limiter_oracle.py)
 //   * _hll_solver / _hllc_solver (+ HLLC-LM)  astronomix/_finite_volume/_riemann_solver/hll.py:33-261
 //   * _lax_friedrichs_solver           astronomix/_finite_volume/_riemann_solver/_lax_friedrichs.py:21-78
 //   * _am_hllc_solver (AM / hybrid)    astronomix/_finite_volume/_riemann_solver/hll.py:264-326
@@ -57,7 +59,7 @@
 namespace {
 
 enum { RS_HLL = 0, RS_HLLC = 1, RS_HLLC_LM = 2, RS_LF = 3, RS_HYBRID = 4, RS_AM = 5 };
-enum { LIM_MINMOD = 0, LIM_OSHER = 1, LIM_DOUBLE_MINMOD = 2, LIM_SUPERBEE = 3, LIM_VAN_ALBADA = 4 };
+enum { LIM_MINMOD = 0, LIM_OSHER = 1, LIM_DOUBLE_MINMOD = 2, LIM_SUPERBEE = 3, LIM_VAN_ALBADA = 4, LIM_VAN_ALBADA_PP = 5 };
 enum { BC_PERIODIC = 0, BC_OPEN = 1, BC_REFLECTIVE = 2 };
 
 template <typename T>
@@ -72,6 +74,8 @@ struct Grid {
   int riemann;       // RS_*
   int limiter;       // LIM_*
   int bc[3][2];      // BC_* per 0-based axis, (low, high) side
+  T C_cfl;           // params.C_cfl (the VAN_ALBADA_PP limiter's q)
+  int mhd;           // config.mhd (the VAN_ALBADA_PP limiter's q)
   int ng() const { return dim + 2; }        // number of gas variables
   int pidx() const { return dim + 1; }      // pressure / energy row
   long shift(long c, int axis, int s) const {
@@ -131,6 +135,8 @@ Grid<T> make_grid(int dim, int nx, int ny, int nz, double gamma, double dx, int 
   g.limiter = limiter;
   for (int a = 0; a < 3; ++a)
     for (int k = 0; k < 2; ++k) g.bc[a][k] = bc ? bc[2 * a + k] : BC_PERIODIC;
+  g.C_cfl = (T)0.4;
+  g.mhd = 0;
   return g;
 }
 
@@ -167,7 +173,8 @@ inline T limit(int lim, T a, T b, T dx) {
       return a * b > 0 ? minmod(maxmod(a, b), minmod(T(2) * a, T(2) * b)) : T(0);
     case LIM_DOUBLE_MINMOD:  // limiters.py:28-32
       return a * b > 0 ? minmod3((a + b) / T(2), T(2) * a, T(2) * b) : T(0);
-    case LIM_VAN_ALBADA: {  // limiters.py:48-64, eps = 3 dx
+    case LIM_VAN_ALBADA:
+    case LIM_VAN_ALBADA_PP: {  // limiters.py:48-64, eps = 3 dx
       const T eps = T(3) * dx, eps2 = T(2) * eps;
       return ((b * b + eps) * a + (a * a + eps) * b) / ((b * b + a * a) + eps2);
     }
@@ -346,6 +353,86 @@ T lf_alpha(const Grid<T>& g, const T* w, int axis) {
   return m;
 }
 
+// Positivity-preserving scaling of the VAN_ALBADA_PP limiter
+// (_finite_volume/_state_evolution/reconstruction.py:168-365) for a gas
+// state w: per cell, alpha[a] (density) and kappa[a] (pressure) per axis and
+// beta (the velocity rows), stored (2 dim + 1, ncell): alpha_0.., kappa_0..,
+// beta. d[a][v] = van Albada gradient along axis a times dx / 2. The weights
+// C = alpha_lax / sum(alpha_lax), alpha_lax[a] = max(|u_a| + c) over the
+// grid, and q = 2 / C_cfl with MHD (the half-step sub-stages), 1 / C_cfl
+// without. A1 keeps the reference's broadcast: the (axis)-shaped product
+// C * alpha * d_rho multiplies the (axis, component) velocity differences
+// along the component index, so A1 = sum_k (sum_a C[k] alpha[k] d_rho[k]
+// d_v[a][k])^2. Periodic boundaries (a ghost cell's factors are not those of
+// the cell it copies).
+template <typename T>
+std::vector<T> pp_factors(const Grid<T>& g, const T* w) {
+  const int dim = g.dim, P = g.pidx();
+  const long nc = g.ncell;
+  T al[3] = {T(0), T(0), T(0)};
+  for (long c = 0; c < nc; ++c) {
+    const T cs = std::sqrt((g.gamma * w[P * nc + c]) / w[c]);
+    for (int a = 0; a < dim; ++a) {
+      const T m = std::fabs(w[(1 + a) * nc + c]) + cs;
+      if (m != m || m > al[a]) al[a] = m;
+    }
+  }
+  T sum = al[0] + al[1];
+  if (dim == 3) sum = sum + al[2];
+  T C[3];
+  for (int a = 0; a < dim; ++a) C[a] = al[a] / sum;
+  const T q = (g.mhd ? T(2) : T(1)) / g.C_cfl;
+  const T eps = T(1e-14), onep = T(1.0 + 1e-14);
+  std::vector<T> fac((size_t)(2 * dim + 1) * nc);
+#pragma omp parallel for schedule(static)
+  for (long c = 0; c < nc; ++c) {
+    T d[3][5];
+    for (int a = 0; a < dim; ++a) {
+      bool fm, fp;
+      const long cm = g.nbr(c, a, -1, &fm), cp = g.nbr(c, a, +1, &fp);
+      for (int v = 0; v <= P; ++v) {
+        const T wm = w[v * nc + cm], w0 = w[v * nc + c], wp = w[v * nc + cp];
+        const T gr = limit(LIM_VAN_ALBADA, (w0 - wm) / g.dx, (wp - w0) / g.dx, g.dx);
+        d[a][v] = (gr * g.dx) / T(2);
+      }
+    }
+    const T rho = w[c], p = w[P * nc + c];
+    T al_d[3], ka_p[3];
+    for (int a = 0; a < dim; ++a) {
+      al_d[a] = std::fabs(d[a][0]) > eps ? std::min(rho / (std::fabs(d[a][0]) * onep), T(1)) : T(1);
+      ka_p[a] = std::fabs(d[a][P]) > eps ? std::min(p / (std::fabs(d[a][P]) * onep), T(1)) : T(1);
+    }
+    T A1 = T(0), A2 = T(0), vsum = T(0);
+    for (int k = 0; k < dim; ++k) {
+      T sk = T(0);
+      for (int a = 0; a < dim; ++a) {
+        const T x = ((C[k] * al_d[k]) * d[k][0]) * d[a][1 + k];
+        sk = a == 0 ? x : sk + x;
+      }
+      A1 = k == 0 ? sk * sk : A1 + sk * sk;
+    }
+    for (int a = 0; a < dim; ++a) {
+      T vs = T(0);
+      for (int k = 0; k < dim; ++k) vs = k == 0 ? d[a][1 + k] * d[a][1 + k] : vs + d[a][1 + k] * d[a][1 + k];
+      A2 = a == 0 ? C[a] * vs : A2 + C[a] * vs;
+      vsum = a == 0 ? vs : vsum + vs;
+    }
+    T beta = T(1);
+    if (vsum > eps) {
+      const T qm2 = q - T(2);
+      const T num = ((qm2 * qm2) * rho) * p;
+      const T den = (g.gamma - T(1)) * (T(2) * A1 + (qm2 * (rho * rho)) * A2);
+      beta = std::min(std::sqrt(num / den), T(1));
+    }
+    for (int a = 0; a < dim; ++a) {
+      fac[(size_t)a * nc + c] = al_d[a];
+      fac[(size_t)(dim + a) * nc + c] = ka_p[a];
+    }
+    fac[(size_t)(2 * dim) * nc + c] = beta;
+  }
+  return fac;
+}
+
 // Compression switch of the AM / hybrid HLLC solvers (hll.py:281-297):
 // g = 1 where sum_d (v_d[i+1] - v_d[i-1]) / (2 dx) < -0.05 c / dx
 template <typename T>
@@ -385,6 +472,10 @@ void gas_stage(const Grid<T>& g, const T* w, T* out, T dt) {
   std::vector<T> F((size_t)ng * nc);
   std::vector<unsigned char> sw;
   if (g.riemann == RS_HYBRID || g.riemann == RS_AM) sw = am_switch(g, w);
+  const bool pp = g.limiter == LIM_VAN_ALBADA_PP;
+  std::vector<T> fac;
+  if (pp) fac = pp_factors(g, w);
+  const int P = g.pidx();
   for (int axis = 1; axis <= g.dim; ++axis) {
     const T alpha = g.riemann == RS_LF ? lf_alpha(g, w, axis) : T(0);
     // interface flux between the cells at offsets s0 and s0 + 1 from c along
@@ -402,8 +493,13 @@ void gas_stage(const Grid<T>& g, const T* w, T* out, T dt) {
         // limited_gradients.py:50-60 ; reconstruction.py:409-429
         const T g_m1 = limit(g.limiter, (b - a) / g.dx, (cc - b) / g.dx, g.dx);
         const T g_0 = limit(g.limiter, (cc - b) / g.dx, (d - cc) / g.dx, g.dx);
-        const T d_m1 = (g_m1 * g.dx) / T(2);
-        const T d_0 = (g_0 * g.dx) / T(2);
+        T d_m1 = (g_m1 * g.dx) / T(2);
+        T d_0 = (g_0 * g.dx) / T(2);
+        if (pp) {  // reconstruction.py:343-365: the differences scaled by the cell's factor
+          const int row = v == 0 ? axis - 1 : (v == P ? g.dim + axis - 1 : 2 * g.dim);
+          d_m1 = d_m1 * fac[(size_t)row * nc + id[1]];
+          d_0 = d_0 * fac[(size_t)row * nc + id[2]];
+        }
         wl[v] = b + d_m1;
         wr[v] = cc - d_0;
       }
@@ -669,15 +765,23 @@ struct OracleConfig {
   int num_timesteps;
   int exact_end_time;
   int max_iter;       // magnetic fixed-point cap (1000 in the reference)
-  int limiter;        // 0 minmod, 1 Osher, 2 double minmod, 3 superbee, 4 van Albada
+  int limiter;        // 0 minmod, 1 Osher, 2 double minmod, 3 superbee, 4 van Albada, 5 VAN_ALBADA_PP
   int bc[6];          // BC_* of x low/high, y low/high, z low/high
   double gamma, dx, C_cfl, t_end, tol;
 };
 
+template <typename T>
+Grid<T> grid_of(const OracleConfig& oc) {
+  Grid<T> g = make_grid<T>(oc.dim, oc.nx, oc.ny, oc.nz, oc.gamma, oc.dx, oc.riemann, oc.limiter, oc.bc);
+  g.C_cfl = (T)oc.C_cfl;
+  g.mhd = oc.mhd;
+  return g;
+}
+
 // loop of _time_integration (time_integration.py:629-801) without snapshots
 template <typename T>
 long time_integration(const OracleConfig& oc, T* state, long max_steps, double* t_out, long* iters_out) {
-  Grid<T> g = make_grid<T>(oc.dim, oc.nx, oc.ny, oc.nz, oc.gamma, oc.dx, oc.riemann, oc.limiter, oc.bc);
+  Grid<T> g = grid_of<T>(oc);
   T t = 0;
   const T t_end = (T)oc.t_end;
   long steps = 0, iters = 0;
@@ -714,23 +818,23 @@ extern "C" {
     return time_integration<T>(*oc, state, max_steps, t_out, iters_out);                           \
   }                                                                                                \
   int oracle_evolve_state_fv_##SFX(const OracleConfig* oc, T* state, double dt) {                  \
-    Grid<T> g = make_grid<T>(oc->dim, oc->nx, oc->ny, oc->nz, oc->gamma, oc->dx, oc->riemann, oc->limiter, oc->bc);     \
+    Grid<T> g = grid_of<T>(*oc);     \
     return evolve_state_fv<T>(g, state, oc->mhd, (T)dt, (T)oc->tol, oc->max_iter, oc->euler);      \
   }                                                                                                \
   double oracle_cfl_dt_##SFX(const OracleConfig* oc, const T* gas) {                               \
-    Grid<T> g = make_grid<T>(oc->dim, oc->nx, oc->ny, oc->nz, oc->gamma, oc->dx, oc->riemann, oc->limiter, oc->bc);     \
+    Grid<T> g = grid_of<T>(*oc);     \
     return (double)cfl_dt<T>(g, gas, (T)oc->C_cfl);                                                \
   }                                                                                                \
   void oracle_gas_stage_##SFX(const OracleConfig* oc, const T* w, T* out, double dt) {             \
-    Grid<T> g = make_grid<T>(oc->dim, oc->nx, oc->ny, oc->nz, oc->gamma, oc->dx, oc->riemann, oc->limiter, oc->bc);     \
+    Grid<T> g = grid_of<T>(*oc);     \
     gas_stage<T>(g, w, out, (T)dt);                                                                \
   }                                                                                                \
   void oracle_gas_rk2_##SFX(const OracleConfig* oc, T* w, double dt) {                             \
-    Grid<T> g = make_grid<T>(oc->dim, oc->nx, oc->ny, oc->nz, oc->gamma, oc->dx, oc->riemann, oc->limiter, oc->bc);     \
+    Grid<T> g = grid_of<T>(*oc);     \
     gas_rk2<T>(g, w, (T)dt);                                                                       \
   }                                                                                                \
   int oracle_magnetic_update_##SFX(const OracleConfig* oc, T* B, T* gas, double dt) {              \
-    Grid<T> g = make_grid<T>(oc->dim, oc->nx, oc->ny, oc->nz, oc->gamma, oc->dx, oc->riemann, oc->limiter, oc->bc);     \
+    Grid<T> g = grid_of<T>(*oc);     \
     return magnetic_update<T>(g, B, gas, (T)dt, (T)oc->tol, oc->max_iter, oc->euler);              \
   }                                                                                                \
   /* interface flux for n independent (wl, wr) pairs laid out (ng, n) */                          \
@@ -757,9 +861,15 @@ extern "C" {
   void oracle_minmod_##SFX(long n, const T* a, const T* b, T* out) {                               \
     for (long i = 0; i < n; ++i) out[i] = minmod<T>(a[i], b[i]);                                   \
   }                                                                                                \
+  /* VAN_ALBADA_PP factors (2 dim + 1, ncell) of a gas state */                                   \
+  void oracle_pp_factors_##SFX(const OracleConfig* oc, const T* w, T* out) {                       \
+    Grid<T> g = grid_of<T>(*oc);                                                                   \
+    const std::vector<T> f = pp_factors<T>(g, w);                                                  \
+    std::copy(f.begin(), f.end(), out);                                                            \
+  }                                                                                                \
   /* compression switch of the AM / hybrid HLLC solvers for a gas state */                        \
   void oracle_am_switch_##SFX(const OracleConfig* oc, const T* w, unsigned char* out) {            \
-    Grid<T> g = make_grid<T>(oc->dim, oc->nx, oc->ny, oc->nz, oc->gamma, oc->dx, oc->riemann, oc->limiter, oc->bc);     \
+    Grid<T> g = grid_of<T>(*oc);     \
     const std::vector<unsigned char> s = am_switch<T>(g, w);                                       \
     std::copy(s.begin(), s.end(), out);                                                            \
   }                                                                                                \

@@ -100,7 +100,9 @@ def test_implemented_path_gate():
         assert unsupported_reason(ok._replace(limiter=lim)) is None
     for rs in (sc.HLLC_LM, sc.LAX_FRIEDRICHS, sc.HYBRID_HLLC, sc.AM_HLLC):
         assert unsupported_reason(ok._replace(riemann_solver=rs)) is None
-    assert "VAN_ALBADA_PP" in unsupported_reason(ok._replace(limiter=sc.VAN_ALBADA_PP))
+    assert unsupported_reason(ok._replace(limiter=sc.VAN_ALBADA_PP)) is None
+    assert "periodic" in unsupported_reason(ok._replace(limiter=sc.VAN_ALBADA_PP, boundary_settings=mixed))
+    assert "limiter" in unsupported_reason(ok._replace(limiter=9))
     assert "Riemann" in unsupported_reason(ok._replace(riemann_solver=7))
     assert "float32" in unsupported_reason(ok._replace(numerical_precision=sc.DOUBLE_PRECISION))
     assert "module" in unsupported_reason(ok._replace(cooling_config=sc.CoolingConfig(cooling=True)))

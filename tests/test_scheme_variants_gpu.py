@@ -1,7 +1,8 @@
 """Scheme variants on the GPU through the C ABI against the CPU restatement
 (oracle/fv_oracle.cpp): the HLLC-LM, hybrid / AM HLLC and Lax-Friedrichs
 fluxes and the Osher,
-double-minmod, superbee and van Albada limiters, in 2D and 3D, hydro and
+double-minmod, superbee, van Albada and positivity-preserving van Albada
+limiters, in 2D and 3D, hydro and
 MHD. Same bar as the default scheme: equal step and fixed-point iteration
 counts, relative L1 per variable <= 1e-5 against the float32 oracle, or, for
 a variable whose own float32 floor is above that, within 3x of that floor
@@ -27,8 +28,26 @@ def rel_l1(a, b):
     return np.array([np.abs(a[v] - b[v]).sum() / max(np.abs(b[v]).sum(), 1e-30) for v in range(a.shape[0])])
 
 
+def fast_flow(shape, mhd, vscale, seed):
+    """Smooth density and pressure with velocities scaled up: the
+    VAN_ALBADA_PP velocity factor beta drops below 1 while its density and
+    pressure factors stay 1 (where those act, a float32 face lands on 0 and
+    the Riemann solver returns NaN, in the reference as here)."""
+    s = W.random_smooth(shape, mhd, seed=seed, amp=0.2).astype(np.float64)
+    s[1:len(shape) + 1] *= vscale
+    return s.astype(np.float32)
+
+
 CASES = [
     # name, builder, dim, mhd, riemann, limiter, C_cfl, t_end
+    ("ot64_vapp", lambda: W.orszag_tang_2d(64), 2, True, sc.HLL, sc.VAN_ALBADA_PP, 0.4, 0.05),
+    ("fast2d_vapp", lambda: fast_flow((48, 80), False, 6.0, 2), 2, False, sc.HLL, sc.VAN_ALBADA_PP, 0.4, 0.01),
+    ("fast2d_mhd_vapp_hllc", lambda: fast_flow((40, 70), True, 6.0, 2), 2, True, sc.HLLC, sc.VAN_ALBADA_PP, 0.4,
+     0.01),
+    ("fast3d_vapp_lf", lambda: fast_flow((12, 10, 72), False, 6.0, 2), 3, False, sc.LAX_FRIEDRICHS,
+     sc.VAN_ALBADA_PP, 0.4, 0.005),
+    ("fast3d_mhd_vapp", lambda: fast_flow((12, 10, 72), True, 2.5, 4), 3, True, sc.HLL, sc.VAN_ALBADA_PP, 0.3,
+     0.004),
     ("ot64_hllclm", lambda: W.orszag_tang_2d(64), 2, True, sc.HLLC_LM, sc.MINMOD, 0.4, 0.05),
     ("ot64_lf", lambda: W.orszag_tang_2d(64), 2, True, sc.LAX_FRIEDRICHS, sc.MINMOD, 0.4, 0.05),
     ("ot64_osher", lambda: W.orszag_tang_2d(64), 2, True, sc.HLL, sc.OSHER, 0.4, 0.05),
@@ -108,6 +127,21 @@ def test_variants_differ_from_the_default(gpu):
                     (sc.HYBRID_HLLC, sc.MINMOD), (sc.AM_HLLC, sc.MINMOD)):
         out, _ = gpu_run(state, W.fv_config(2, 64, mhd=True, riemann=rs)._replace(limiter=lim), params)
         assert np.abs(out - base).max() > 1e-6, (rs, lim)
+
+
+def test_pp_scaling_reaches_the_kernels(gpu, oracle):
+    """On a flow whose velocity factor acts, VAN_ALBADA_PP differs from plain
+    van Albada on the GPU by what the oracle says it should."""
+    state = fast_flow((48, 80), False, 6.0, 2)
+    params = SimulationParams(C_cfl=0.4, t_end=0.01, gamma=5.0 / 3.0)
+    cfg = W.fv_config(2, 80, mhd=False, riemann=sc.HLL)
+    pp, _ = gpu_run(state, cfg._replace(limiter=sc.VAN_ALBADA_PP), params)
+    va, _ = gpu_run(state, cfg._replace(limiter=sc.VAN_ALBADA), params)
+    got = rel_l1(pp, va).max()
+    ref_pp, *_ = oracle_run(oracle, state, cfg._replace(limiter=sc.VAN_ALBADA_PP), params, np.float32)
+    ref_va, *_ = oracle_run(oracle, state, cfg._replace(limiter=sc.VAN_ALBADA), params, np.float32)
+    want = rel_l1(ref_pp, ref_va).max()
+    assert want > 1e-3 and abs(got - want) <= 0.05 * want
 
 
 def test_adjoint_refuses_variants(gpu):
