@@ -348,14 +348,16 @@ __device__ __forceinline__ void riemann(const float* wl, const float* wr, float*
     flux_hll<DIM, AX>(wl, wr, f, ph);
 }
 
-enum { SRC_PLAIN = 0, SRC_MAGFIX = 1 };
+enum { SRC_PLAIN = 0, SRC_MAGFIX = 1, SRC_PFIX = 2 };
 
 // Primitive gas state at one cell, in two halves so that loads can be issued
 // a plane ahead of their use: load_raw fetches the fields, derive_w forms W.
 // SRC_PLAIN reads the NV gas rows. SRC_MAGFIX reads (rho, v0, p) of the gas
 // state entering the magnetic sub-step plus the converged velocity vn (slots
 // 5..7) and applies the sub-step's write-back: v <- vn and the energy fix with
-// gamma = 5/3 (_magnetic_field_update.py:208-240).
+// gamma = 5/3 (_magnetic_field_update.py:208-240). SRC_PFIX (Heun centre
+// only) reads the same fixed state ready-made: rho, the converged velocity
+// and the fixed pressure that the SRC_MAGFIX stage stored (StageArgs::pfix).
 // Element c of a field through a 32-bit byte offset (c < 2^30): the load
 // takes the scalar base plus one 32-bit vector offset, no 64-bit address math.
 __device__ __forceinline__ float ld32(const float* base, unsigned c) {
@@ -365,13 +367,13 @@ __device__ __forceinline__ void st32(float* base, unsigned c, float v) { *(float
 
 template <int DIM, int MODE>
 struct Raw {
-  static constexpr int N = MODE == SRC_PLAIN ? DIM + 2 : 8;
+  static constexpr int N = MODE != SRC_MAGFIX ? DIM + 2 : 8;
   float f[N];
 };
 
 template <int DIM, int MODE>
 __device__ __forceinline__ void load_raw(const float* const* s, unsigned c, Raw<DIM, MODE>& r) {
-  if (MODE == SRC_PLAIN) {
+  if (MODE != SRC_MAGFIX) {
 #pragma unroll
     for (int v = 0; v < DIM + 2; ++v) r.f[v] = ld32(s[v], c);
   } else {
@@ -383,7 +385,7 @@ __device__ __forceinline__ void load_raw(const float* const* s, unsigned c, Raw<
 template <int DIM, int MODE>
 __device__ __forceinline__ void derive_w(const Raw<DIM, MODE>& r, float* w) {
   constexpr int NV = DIM + 2, P = DIM + 1;
-  if (MODE == SRC_PLAIN) {
+  if (MODE != SRC_MAGFIX) {
 #pragma unroll
     for (int v = 0; v < NV; ++v) w[v] = r.f[v];
   } else {
@@ -444,6 +446,9 @@ struct StageArgs {
   long ppstride;
   float ppq;             // q = 2 / C_cfl with MHD, 1 / C_cfl without
   int pp;                // the kernels instantiated with LIMX scale their slopes by ppf
+  // energy-fixed pressure of the magnetic sub-step's output: written by the
+  // SRC_MAGFIX stage (when set), read by the W0M == SRC_PFIX Heun centre
+  float* pfix;
 };
 
 // One unsplit forward-Euler stage (evolve_state.py:338-414), optionally with
@@ -497,6 +502,11 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
   if (W0M == SRC_MAGFIX) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) w0p[5 + k] = a.selv[par][k] + so;
+  }
+  if (W0M == SRC_PFIX) {
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) w0p[1 + k] = a.selv[par][k] + so;
+    w0p[DIM + 1] = a.pfix + so;
   }
   float* outp[5];
 #pragma unroll
@@ -629,8 +639,9 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
   bool fq;  // unused: the flip of a prefetched plane is recomputed when it is derived
   if (owner && DIM == 3) load_raw<DIM, SRC>(src, xplane(x0 + 2, fq) + col, rq);
   if (h >= 0) load_raw<DIM, SRC>(src, (DIM == 3 ? (unsigned)x0 * sx : 0u) + hoff, rh);
-  // the 8-field energy-fix centre is not prefetched (register pressure)
-  constexpr bool PRE0 = HEUN && W0M == SRC_PLAIN;
+  // the 8-field SRC_MAGFIX centre would not be prefetched (register pressure):
+  // the Heun stage after the magnetic sub-step takes the 5-field SRC_PFIX form
+  constexpr bool PRE0 = HEUN && W0M != SRC_MAGFIX;
   if (PRE0 && active) load_raw<DIM, W0M>(w0p, (DIM == 3 ? (unsigned)x0 * sx : 0u) + col, r0);
 
   float mx[3] = {0.f, 0.f, 0.f};
@@ -768,6 +779,7 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
       }
 #pragma unroll
       for (int v = 0; v < NV; ++v) st32(outp[v], cell, W1[v]);
+      if (SRC == SRC_MAGFIX && a.pfix) st32(a.pfix + so, cell, q0[P]);
       if (a.want_cfl) {
         const float cs = fsqrt(ph.gamma * W1[P] * frcp(W1[0]));
 #pragma unroll
@@ -1815,7 +1827,7 @@ amx_status validate(const amx_fv_config* c) {
 }
 
 // G1 (gas rows) | P0 | P1 | P2 (B, v iterates of the fixed point)
-long ws_fields(const amx_fv_config* cfg) { return (cfg->dim + 2) + (cfg->mhd ? 18 : 0); }
+long ws_fields(const amx_fv_config* cfg) { return (cfg->dim + 2) + (cfg->mhd ? 19 : 0); }
 long state_fields(const amx_fv_config* cfg) { return (cfg->dim + 2) + (cfg->mhd ? 3 : 0); }
 long cells(const amx_fv_config* cfg) { return (long)cfg->nx * cfg->ny * (cfg->dim == 3 ? cfg->nz : 1); }
 
@@ -1959,6 +1971,7 @@ struct Bufs {
   float* B[3];
   float* G1;
   float* P[3][6];  // launch k of the fixed point writes P[k % 3]
+  float* PF;       // energy-fixed pressure of the magnetic sub-step's output (MHD)
 };
 
 Bufs bufs(const amx_fv_ctx* c, float* state) {
@@ -1974,6 +1987,7 @@ Bufs bufs(const amx_fv_ctx* c, float* state) {
   if (c->cfg.mhd)
     for (int j = 0; j < 3; ++j)
       for (int k = 0; k < 6; ++k) b.P[j][k] = ws + (long)(c->NV + 6 * j + k) * n;
+  if (c->cfg.mhd) b.PF = ws + (long)(c->NV + 18) * n;
   return b;
 }
 
@@ -2281,6 +2295,7 @@ amx_status step_phase3(amx_fv_ctx* c, const Bufs& b, int nspan, hipStream_t st, 
     b1.src[v] = b.gas[v];
     b1.out[v] = b.G1 + (long)v * c->ncell;
   }
+  b1.pfix = b.PF;
   for (int j = 0; j < 3; ++j)
     for (int k = 0; k < 3; ++k) {
       b1.selv[j][k] = b.P[j][3 + k];
@@ -2303,8 +2318,9 @@ amx_status step_phase3(amx_fv_ctx* c, const Bufs& b, int nspan, hipStream_t st, 
   for (int k = 0; k < 3; ++k) b2.bdst[k] = b.B[k];
   b2.copy_b = 1;
   b2.want_cfl = want_cfl;
+  b2.pfix = b.PF;
   e = time_begin(c, st);
-  HIPCHK((launch_stage<SRC_PLAIN, true, SRC_MAGFIX>(c, b2, nspan, st)));
+  HIPCHK((launch_stage<SRC_PLAIN, true, SRC_PFIX>(c, b2, nspan, st)));
   time_end(c, st, e, CLS_STAGE, 0);
   return AMX_OK;
 }

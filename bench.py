@@ -30,10 +30,11 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E datasheet peak
 #             iterates (12) -> 19 fields; later pairs also read the input iterate
 #             (6) -> 25 fields
 #   k_stage : mean over the four launches of a step
-#             A1 (R5 W5) + A2 (R10 W5) + A1' (R8 W5) + A2' (R16 W8) = 62 fields -> 62 B
+#             A1 (R5 W5) + A2 (R10 W5) + A1' (R8 W6: + the fixed pressure)
+#             + A2' (R13 W8: stencil 5, Heun centre 5, B 3) = 60 fields -> 60 B
 MAG_FIELDS_FIRST, MAG_FIELDS_LATER = 19.0, 25.0
-STAGE_BYTES_PER_CELL = 62.0
-STEP_FIELDS_STAGES = 62
+STAGE_BYTES_PER_CELL = 60.0
+STEP_FIELDS_STAGES = 60
 
 
 def cpu_baseline(n: int, steps: int):
