@@ -70,7 +70,18 @@ struct Ctl {
   unsigned cflp[3][NSLOT];      // CFL partial maxima
   unsigned resp[NRES][NSLOT];   // fixed-point residual partials, by launch index % NRES
   unsigned lfp[3][NSLOT];       // Lax-Friedrichs: partial maxima of |u| + c of the stage input
+  int bset;                     // iterate set holding B0 of the magnetic sub-step; -1: the state's B rows
 };
+
+// Iterate set of fixed-point launch k: the three sets other than bset
+// (with bset = -1, sets 0, 1, 2 in turn). A loop that keeps the final B in
+// its set instead of copying it into the state moves bset there at the end
+// of the step, so the next sub-step's launches write around it.
+__host__ __device__ inline int iterate_set(int bset, int k) {
+  const int ex = bset < 0 ? 3 : bset;
+  const int j = k % 3;
+  return j < ex ? j : j + 1;
+}
 
 struct Phys {
   float gamma;    // params.gamma
@@ -421,11 +432,12 @@ struct StageArgs {
   const float* w0[8];    // Heun: stage-0 state at the cell centre
   float* out[5];         // output gas rows
   float* bdst[3];        // optional copy of the final B of the magnetic sub-step
-  // final fixed-point iterate of the slot, chosen on the device by its launch
-  // index K mod 3: v_K replaces src[5..7] (SRC_MAGFIX) and w0[5..7]
-  // (W0M == SRC_MAGFIX), B_K is the copy source
-  const float* selv[3][3];
-  const float* selb[3][3];
+  // iterate sets of the fixed point (set 3 only in single-simulation
+  // contexts); the final iterate's set is chosen on the device from its launch
+  // index K (iterate_set): v_K replaces src[5..7] (SRC_MAGFIX) and the Heun
+  // centre's velocity (SRC_PFIX), B_K is the copy source
+  const float* selv[4][3];
+  const float* selb[4][3];
   Ctl* ctl;
   const Phys* phs;
   long sstride;
@@ -487,7 +499,7 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
   Ctl* const C = a.ctl + slot;
   if (!C->run) return;  // block-uniform: empty or finished slot
   const long so = slot * a.sstride;
-  const int par = C->iters % 3;  // buffer of the fixed point's final iterate
+  const int par = iterate_set(C->bset, C->iters);  // set of the fixed point's final iterate
   const float* src[8];
   const float* w0p[8];
 #pragma unroll
@@ -832,7 +844,7 @@ __global__ __launch_bounds__(256) void k_lf_alpha(const StageArgs a) {
   Ctl* const C = a.ctl + slot;
   if (!C->run) return;  // block-uniform
   const long so = slot * a.sstride;
-  const int par = C->iters % 3;
+  const int par = iterate_set(C->bset, C->iters);
   const float* src[8];
 #pragma unroll
   for (int v = 0; v < 8; ++v) src[v] = a.src[v] ? a.src[v] + so : nullptr;
@@ -872,7 +884,7 @@ __global__ __launch_bounds__(256) void k_am_switch(const StageArgs a) {
   Ctl* const C = a.ctl + slot;
   if (!C->run) return;  // block-uniform
   const long so = slot * a.sstride;
-  const int par = C->iters % 3;
+  const int par = iterate_set(C->bset, C->iters);
   const float* src[8];
 #pragma unroll
   for (int v = 0; v < 8; ++v) src[v] = a.src[v] ? a.src[v] + so : nullptr;
@@ -929,7 +941,7 @@ __global__ __launch_bounds__(256) void k_pp_factors(const StageArgs a) {
   Ctl* const C = a.ctl + slot;
   if (!C->run) return;  // block-uniform
   const long so = slot * a.sstride;
-  const int par = C->iters % 3;
+  const int par = iterate_set(C->bset, C->iters);
   const float* src[8];
 #pragma unroll
   for (int v = 0; v < 8; ++v) src[v] = a.src[v] ? a.src[v] + so : nullptr;
@@ -1020,6 +1032,8 @@ struct MagArgs {
   const float* B0[3];
   const float* v0[3];  // v0[2] == nullptr in 2D (vz = 0)
   const float* rho;
+  float* PS[4][6];     // iterate sets (B rows 0..2, v rows 3..5); set 3 may be absent
+  // slot-local pointers, filled on the device from PS and ctl->bset:
   const float* Bk[3];  // input iterate: output of launch L-1 (the state's B, v for L = 0)
   const float* vk[3];  // vk[2] == nullptr: vz = 0
   float* Bo[2][3];     // outputs of launches L and L+1
@@ -1211,17 +1225,21 @@ __global__ __launch_bounds__(((TA + 4) * (TBM + 4) + 255) / 256 * 256) void k_ma
   }
   const long so = slot * a.sstride;
   MagArgs m = a;  // slot-local pointers
+  // B0 is in the state's rows or, in a loop that keeps B in the iterate
+  // sets, in set bset; launch k writes set iterate_set(bset, k)
+  const int bs = C->bset;
+  const int sin_ = L > 0 ? iterate_set(bs, L - 1) : 0;
+  const int so0 = iterate_set(bs, L), so1 = iterate_set(bs, L + 1);
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
-    m.B0[i] = a.B0[i] + so;
-    m.Bk[i] = a.Bk[i] + so;
+    m.B0[i] = (bs < 0 ? a.B0[i] : a.PS[bs][i]) + so;
     m.v0[i] = a.v0[i] ? a.v0[i] + so : nullptr;
-    m.vk[i] = a.vk[i] ? a.vk[i] + so : nullptr;
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      m.Bo[j][i] = a.Bo[j][i] + so;
-      m.vo[j][i] = a.vo[j][i] + so;
-    }
+    m.Bk[i] = L > 0 ? a.PS[sin_][i] + so : m.B0[i];
+    m.vk[i] = L > 0 ? a.PS[sin_][3 + i] + so : m.v0[i];
+    m.Bo[0][i] = a.PS[so0][i] + so;
+    m.vo[0][i] = a.PS[so0][3 + i] + so;
+    m.Bo[1][i] = a.PS[so1][i] + so;
+    m.vo[1][i] = a.PS[so1][3 + i] + so;
   }
   m.rho = a.rho + so;
   const int bid = swizzle_block(blockIdx.x, gridDim.x);
@@ -1595,16 +1613,39 @@ __global__ void k_step_begin(Ctl* ctl, StepParams p) {
 __global__ void k_set_dt(Ctl* c, const float* dtp) {
   c->dt = *dtp;
   c->run = 1;
+  c->bset = -1;  // one step of a caller's state: B0 is its B rows
   zero_cflp(c);
   fixed_point_reset(c);
 }
 
-__global__ void k_step_end(Ctl* ctl) {
+// keep_b: the final B of the step stays in its iterate set, which holds B0
+// of the next step (the state's B rows are written once, by k_copy_bset).
+__global__ void k_step_end(Ctl* ctl, int keep_b) {
   Ctl* const c = ctl + blockIdx.x;
   if (!c->run) return;
   c->t = c->t + c->dt;
   c->steps += 1;
+  if (keep_b) c->bset = iterate_set(c->bset, c->iters);
 }
+
+// B of slot 0 from its iterate set into the state's B rows (a loop that kept
+// B in the sets), then k_bset_clear hands B0 back to the state's rows.
+struct BsetArgs {
+  const float* PB[4][3];  // B rows of the iterate sets
+  float* dst[3];
+  long n;
+};
+
+__global__ void k_copy_bset(const Ctl* ctl, BsetArgs a) {
+  const int bs = ctl->bset;
+  if (bs < 0) return;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += (long)gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) a.dst[k][i] = a.PB[bs][k][i];
+  }
+}
+
+__global__ void k_bset_clear(Ctl* ctl) { ctl->bset = -1; }
 
 __global__ void k_cfl_out(Ctl* c, float C, float dx, int dim, float* out) {
   cfl_reduce(c, dim);
@@ -1629,6 +1670,7 @@ __global__ void k_slot_reset(Ctl* ctl, int slot0, float t_end, int live) {
   c->live = live;
   c->run = live;
   c->t_end = t_end;
+  c->bset = -1;
 }
 
 // Streaming copy, 4 floats per lane per iteration.
@@ -1719,6 +1761,7 @@ struct amx_fv_ctx {
   void* hook_user;
   void* snapwork;   // snapshot diagnostics partials (allocated on first use)
   int klast;        // pool: last fixed-point launch of the step in flight
+  int keep_b;       // the running loop keeps B in the iterate sets (no per-step copy)
   int use_graphs;   // pool: replay captured HIP graphs of the step halves
   std::map<std::pair<int, int>, hipGraphExec_t> g_launch;  // (span, klast)
   std::map<int, hipGraphExec_t> g_finish;                  // span
@@ -1827,7 +1870,11 @@ amx_status validate(const amx_fv_config* c) {
 }
 
 // G1 (gas rows) | P0 | P1 | P2 (B, v iterates of the fixed point)
-long ws_fields(const amx_fv_config* cfg) { return (cfg->dim + 2) + (cfg->mhd ? 19 : 0); }
+// G1 (gas rows) | P0 | P1 | P2 (B, v iterate sets) | PF (fixed pressure) | P3
+// (the fourth set, single simulations: the loop keeps B in the sets)
+long ws_fields(const amx_fv_config* cfg, int own_state) {
+  return (cfg->dim + 2) + (cfg->mhd ? 19 + (own_state ? 0 : 6) : 0);
+}
 long state_fields(const amx_fv_config* cfg) { return (cfg->dim + 2) + (cfg->mhd ? 3 : 0); }
 long cells(const amx_fv_config* cfg) { return (long)cfg->nx * cfg->ny * (cfg->dim == 3 ? cfg->nz : 1); }
 
@@ -1919,7 +1966,7 @@ amx_status make_ctx(const amx_fv_config* cfg, int cap, int own_state, amx_fv_ctx
   c->ncx_mag = (c->nxm + c->lx_mag - 1) / c->lx_mag;
   c->cap = cap;
   c->own_state = own_state;
-  const long per_slot = c->ncell * (ws_fields(cfg) + (own_state ? c->NS : 0));
+  const long per_slot = c->ncell * (ws_fields(cfg, own_state) + (own_state ? c->NS : 0));
   c->sstride = own_state ? per_slot : 0;
   if (hipMalloc((void**)&c->mem, sizeof(float) * (size_t)per_slot * cap) != hipSuccess) {
     c->mem = nullptr;
@@ -1970,7 +2017,7 @@ struct Bufs {
   float* gas[5];
   float* B[3];
   float* G1;
-  float* P[3][6];  // launch k of the fixed point writes P[k % 3]
+  float* P[4][6];  // iterate sets: launch k writes P[iterate_set(bset, k)]; P[3] single simulations only
   float* PF;       // energy-fixed pressure of the magnetic sub-step's output (MHD)
 };
 
@@ -1988,6 +2035,8 @@ Bufs bufs(const amx_fv_ctx* c, float* state) {
     for (int j = 0; j < 3; ++j)
       for (int k = 0; k < 6; ++k) b.P[j][k] = ws + (long)(c->NV + 6 * j + k) * n;
   if (c->cfg.mhd) b.PF = ws + (long)(c->NV + 18) * n;
+  if (c->cfg.mhd && !c->own_state)
+    for (int k = 0; k < 6; ++k) b.P[3][k] = ws + (long)(c->NV + 19 + k) * n;
   return b;
 }
 
@@ -2141,7 +2190,8 @@ hipError_t launch_cfl(const amx_fv_ctx* c, const float* state, int slot0, int ns
 }
 
 // Launches the fixed-point pair (L, L+1): input = launch L-1's output (the
-// state's B and gas velocity for L = 0), outputs to P[L % 3], P[(L+1) % 3].
+// state's B and gas velocity for L = 0); the kernel picks the iterate sets
+// from ctl->bset (iterate_set).
 hipError_t launch_mag_pair(amx_fv_ctx* c, const Bufs& b, int L, int nspan, hipStream_t st) {
   const int dim = c->cfg.dim;
   MagArgs m = mag_base(c);
@@ -2150,22 +2200,8 @@ hipError_t launch_mag_pair(amx_fv_ctx* c, const Bufs& b, int L, int nspan, hipSt
   m.v0[1] = b.gas[2];
   m.v0[2] = dim == 3 ? b.gas[3] : nullptr;
   m.rho = b.gas[0];
-  if (L == 0) {
-    for (int k = 0; k < 3; ++k) {
-      m.Bk[k] = m.B0[k];
-      m.vk[k] = m.v0[k];
-    }
-  } else {
-    for (int k = 0; k < 3; ++k) {
-      m.Bk[k] = b.P[(L - 1) % 3][k];
-      m.vk[k] = b.P[(L - 1) % 3][3 + k];
-    }
-  }
-  for (int j = 0; j < 2; ++j)
-    for (int k = 0; k < 3; ++k) {
-      m.Bo[j][k] = b.P[(L + j) % 3][k];
-      m.vo[j][k] = b.P[(L + j) % 3][3 + k];
-    }
+  for (int j = 0; j < 4; ++j)
+    for (int k = 0; k < 6; ++k) m.PS[j][k] = b.P[j][k];
   m.L = L;
   const int e = time_begin(c, st);
   const hipError_t err = launch_mag(c, m, nspan, st);
@@ -2296,7 +2332,7 @@ amx_status step_phase3(amx_fv_ctx* c, const Bufs& b, int nspan, hipStream_t st, 
     b1.out[v] = b.G1 + (long)v * c->ncell;
   }
   b1.pfix = b.PF;
-  for (int j = 0; j < 3; ++j)
+  for (int j = 0; j < 4; ++j)
     for (int k = 0; k < 3; ++k) {
       b1.selv[j][k] = b.P[j][3 + k];
       b1.selb[j][k] = b.P[j][k];
@@ -2310,13 +2346,14 @@ amx_status step_phase3(amx_fv_ctx* c, const Bufs& b, int nspan, hipStream_t st, 
     b2.w0[v] = b.gas[v];
     b2.out[v] = b.gas[v];
   }
-  for (int j = 0; j < 3; ++j)
+  for (int j = 0; j < 4; ++j)
     for (int k = 0; k < 3; ++k) {
       b2.selv[j][k] = b.P[j][3 + k];
       b2.selb[j][k] = b.P[j][k];
     }
+  // the final B into the state, unless the loop keeps it in its iterate set
   for (int k = 0; k < 3; ++k) b2.bdst[k] = b.B[k];
-  b2.copy_b = 1;
+  b2.copy_b = c->keep_b ? 0 : 1;
   b2.want_cfl = want_cfl;
   b2.pfix = b.PF;
   e = time_begin(c, st);
@@ -2354,7 +2391,7 @@ extern "C" amx_status amx_fv_workspace_bytes(const amx_fv_config* cfg, size_t* b
   amx_status s = validate(cfg);
   if (s != AMX_OK) return s;
   if (!bytes) return fail(AMX_ERR_INVALID, "null output%s");
-  *bytes = sizeof(float) * (size_t)cells(cfg) * ws_fields(cfg) + sizeof(Ctl) + sizeof(Phys);
+  *bytes = sizeof(float) * (size_t)cells(cfg) * ws_fields(cfg, 0) + sizeof(Ctl) + sizeof(Phys);
   if (cfg->riemann == AMX_HYBRID_HLLC || cfg->riemann == AMX_AM_HLLC) *bytes += (size_t)cells(cfg);
   if (cfg->limiter == AMX_VAN_ALBADA_PP) *bytes += sizeof(float) * (size_t)cells(cfg) * (2 * cfg->dim + 1);
   return AMX_OK;
@@ -2464,6 +2501,15 @@ amx_status fv_loop(amx_fv_ctx* c, float* state, const amx_fv_run* run, snap::Dri
                    hipStream_t st) {
   hipLaunchKernelGGL(k_slot_reset, dim3(1), dim3(1), 0, st, c->ctl, 0, run->t_end, 1);
   HIPCHK(hipGetLastError());
+  // Without snapshot rows or a step hook nothing reads the state's B rows
+  // between steps: the final B of each step stays in its iterate set (the
+  // next step's B0) and is copied into the state once, after the loop.
+  struct KeepB {
+    amx_fv_ctx* c;
+    ~KeepB() { c->keep_b = 0; }
+  } keep_guard{c};
+  c->keep_b = c->cfg.mhd && !snp && !c->hook ? 1 : 0;
+  const Bufs bl = bufs(c, state);
   if (!run->fixed_timestep) HIPCHK(launch_cfl(c, state, 0, 1, st));
   StepParams sp;
   sp.fixed = run->fixed_timestep;
@@ -2497,7 +2543,7 @@ amx_status fv_loop(amx_fv_ctx* c, float* state, const amx_fv_run* run, snap::Dri
     int K = 0;
     amx_status s = do_step(c, state, st, !run->fixed_timestep, &K);
     if (s != AMX_OK) return s;
-    hipLaunchKernelGGL(k_step_end, dim3(1), dim3(1), 0, st, c->ctl);
+    hipLaunchKernelGGL(k_step_end, dim3(1), dim3(1), 0, st, c->ctl, c->keep_b);
     HIPCHK(hipGetLastError());
     if (!c->cfg.mhd) {
       // hydro steps have no fixed-point readback: read dt here
@@ -2516,6 +2562,19 @@ amx_status fv_loop(amx_fv_ctx* c, float* state, const amx_fv_run* run, snap::Dri
       s = snp->end(t, state, st);
       if (s != AMX_OK) return s;
     }
+  }
+  if (c->keep_b) {
+    BsetArgs ba;
+    for (int j = 0; j < 4; ++j)
+      for (int k = 0; k < 3; ++k) ba.PB[j][k] = bl.P[j][k];
+    for (int k = 0; k < 3; ++k) ba.dst[k] = bl.B[k];
+    ba.n = c->ncell;
+    long blocks = (c->ncell + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_copy_bset, dim3(blocks), dim3(256), 0, st, c->ctl, ba);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_bset_clear, dim3(1), dim3(1), 0, st, c->ctl);
+    HIPCHK(hipGetLastError());
   }
   HIPCHK(hipMemcpyAsync(c->hctl, c->ctl, offsetof(Ctl, cflp), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
@@ -2713,7 +2772,7 @@ extern "C" amx_status amx_fv_pool_slot_bytes(const amx_fv_config* cfg, size_t* b
   amx_status s = validate(cfg);
   if (s != AMX_OK) return s;
   if (!bytes) return fail(AMX_ERR_INVALID, "null output%s");
-  *bytes = sizeof(float) * (size_t)cells(cfg) * (ws_fields(cfg) + state_fields(cfg)) + sizeof(Ctl) + sizeof(Phys);
+  *bytes = sizeof(float) * (size_t)cells(cfg) * (ws_fields(cfg, 1) + state_fields(cfg)) + sizeof(Ctl) + sizeof(Phys);
   if (cfg->riemann == AMX_HYBRID_HLLC || cfg->riemann == AMX_AM_HLLC) *bytes += (size_t)cells(cfg);
   if (cfg->limiter == AMX_VAN_ALBADA_PP) *bytes += sizeof(float) * (size_t)cells(cfg) * (2 * cfg->dim + 1);
   return AMX_OK;
@@ -2861,7 +2920,7 @@ extern "C" amx_status amx_fv_pool_step_finish(amx_fv_pool* p, void* stream) {
   amx_status s = replay_or_capture(c, st, &x, x != nullptr, [&]() -> amx_status {
     amx_status r = step_phase3(c, b, span, st, !c->sp.fixed);
     if (r != AMX_OK) return r;
-    hipLaunchKernelGGL(k_step_end, dim3(span), dim3(1), 0, st, c->ctl);
+    hipLaunchKernelGGL(k_step_end, dim3(span), dim3(1), 0, st, c->ctl, 0);
     HIPCHK(hipGetLastError());
     return AMX_OK;
   });

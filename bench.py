@@ -31,10 +31,11 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E datasheet peak
 #             (6) -> 25 fields
 #   k_stage : mean over the four launches of a step
 #             A1 (R5 W5) + A2 (R10 W5) + A1' (R8 W6: + the fixed pressure)
-#             + A2' (R13 W8: stencil 5, Heun centre 5, B 3) = 60 fields -> 60 B
+#             + A2' (R10 W5: stencil 5, Heun centre 5; the loop keeps B in the
+#             fixed point's iterate sets, no per-step copy) = 54 fields -> 54 B
 MAG_FIELDS_FIRST, MAG_FIELDS_LATER = 19.0, 25.0
-STAGE_BYTES_PER_CELL = 60.0
-STEP_FIELDS_STAGES = 60
+STAGE_BYTES_PER_CELL = 54.0
+STEP_FIELDS_STAGES = 54
 
 
 def cpu_baseline(n: int, steps: int):

@@ -217,3 +217,35 @@ def test_blast_256_invariants(gpu):
     err = rel_l1(mirror, out)
     assert err.max() <= 1e-5, err
     assert np.abs(out[4] - state[4]).max() > 1.0  # the blast has moved
+
+
+@pytest.mark.parametrize("case", ["blast3d", "ot2d"])
+def test_loop_keeping_b_in_the_iterate_sets_equals_stepwise_copies(gpu, case):
+    """The device-resident loop keeps each step's final B in the fixed
+    point's iterate set that the next step reads as B0, and writes B into the
+    state once after the last step; amx_fv_evolve_state copies B into the
+    state every step. The same fixed-dt steps give the same bits."""
+    import torch
+    print(f"[gpu] loop vs stepwise B {case}", flush=True)
+    if case == "blast3d":
+        state, dim, n, C = W.blast_3d(32), 3, 32, 0.3
+    else:
+        state, dim, n, C = W.orszag_tang_2d(96), 2, 96, 0.4
+    cfg = W.fv_config(dim, n, mhd=True, riemann=HLL)
+    t_end, steps = 0.006, 7
+    ctx = _native.fv_context(native_config(state.shape, cfg, 5.0 / 3.0, C))
+    s_loop = torch.from_numpy(np.ascontiguousarray(state)).cuda()
+    run = _native.AmxFvRun(t_end=t_end, fixed_timestep=1, num_timesteps=steps, exact_end_time=1, max_steps=-1)
+    st = ctx.time_integration(s_loop, run)
+    assert st.steps == steps and st.fixed_point_iterations > steps
+    s_step = torch.from_numpy(np.ascontiguousarray(state)).cuda()
+    dt = torch.tensor(np.float32(t_end) / np.float32(steps), dtype=torch.float32, device="cuda")
+    iters = sum(ctx.evolve_state(s_step, dt) for _ in range(steps))
+    torch.cuda.synchronize()
+    assert iters == st.fixed_point_iterations
+    assert torch.equal(s_loop, s_step)
+    # and a second loop on the same context starts again from the state's B
+    s_again = torch.from_numpy(np.ascontiguousarray(state)).cuda()
+    ctx.time_integration(s_again, run)
+    torch.cuda.synchronize()
+    assert torch.equal(s_again, s_loop)
