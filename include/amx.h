@@ -100,8 +100,11 @@ typedef struct amx_fv_stats {
 
 typedef struct amx_fv_ctx amx_fv_ctx;
 
-/* Creates a context on the current HIP device and allocates its workspace
- * (one stage buffer of the gas rows and two magnetic iterate buffers).
+/* Creates a context on the current HIP device and allocates its workspace:
+ * one stage buffer of the gas rows and, with MHD, four sets of fixed-point
+ * iterates (B, v) plus the energy-fixed pressure (amx_fv_workspace_bytes).
+ * The step loop keeps each step's final B in its iterate set and writes the
+ * state's B rows once, at the end of the loop.
  * Replaces the jit/compile setup of time_integration (time_integration.py:108-140). */
 amx_status amx_fv_create(const amx_fv_config* cfg, amx_fv_ctx** out);
 
