@@ -163,3 +163,35 @@ def test_am_switch_fires_in_the_blast(gpu, oracle):
     out, _ = gpu_run(state, W.fv_config(3, 32, mhd=True, riemann=sc.HLLC), params)
     sw = oracle.am_switch(np.ascontiguousarray(out[:5]), oc)
     assert 0 < sw.sum() < sw.size
+
+
+@pytest.mark.parametrize("rs,lim", [(sc.LAX_FRIEDRICHS, sc.OSHER), (sc.AM_HLLC, sc.MINMOD),
+                                    (sc.HLL, sc.VAN_ALBADA_PP)])
+def test_pool_slots_equal_single_runs_with_variants(gpu, rs, lim):
+    """The slot-batched pool, which replays captured HIP graphs of its step
+    halves, runs the variants' pre-pass kernels (Lax-Friedrichs maxima,
+    compression switch, positivity factors) per slot: every slot equals its
+    own single-simulation run bit for bit, over tens of replays."""
+    import torch
+    print(f"[gpu] pool variants rs={rs} lim={lim}", flush=True)
+    states = [fast_flow((40, 70), True, 3.0, 2), W.random_smooth((40, 70), True, seed=5, amp=0.3)]
+    cfg = W.fv_config(2, 70, mhd=True, riemann=rs)._replace(limiter=lim)
+    params = SimulationParams(C_cfl=0.4, t_end=0.01, gamma=5.0 / 3.0)
+    singles = [gpu_run(s, cfg, params) for s in states]
+    pool = _native.FvPool(native_config(states[0].shape, cfg, params.gamma, params.C_cfl), 2)
+    try:
+        for i, s in enumerate(states):
+            pool.load(i, s, params.gamma, params.t_end)
+        done = {}
+        while len(done) < 2:
+            pool.step_launch()
+            pool.step_finish()
+            for i in range(2):
+                if i not in done and pool.status(i).finished:
+                    done[i] = pool.status(i)
+        for i, (ref, st) in enumerate(singles):
+            out = pool.read(i, torch.empty(states[i].shape, dtype=torch.float32, device="cuda")).cpu().numpy()
+            assert done[i].steps == st.steps > 5
+            np.testing.assert_array_equal(out, ref)
+    finally:
+        pool.close()
