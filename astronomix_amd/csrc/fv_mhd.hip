@@ -1,12 +1,15 @@
 // MI355X (gfx950) kernels and C ABI of the finite-volume MHD step.
 //
 // One kernel per global-synchronisation interval of the reference dataflow:
-//   * k_stage   one forward-Euler unsplit gas stage (minmod reconstruction +
-//               HLL/HLLC fluxes + flux-divergence update), optionally fused with
+//   * k_stage   one forward-Euler unsplit gas stage (limited reconstruction +
+//               Riemann fluxes + flux-divergence update), optionally fused with
 //               the SSP-RK2 Heun average, the Pang-Wu energy fix on its inputs,
-//               the CFL maxima of its output and the copy of the final B.
+//               the CFL maxima of its output and (outside the step loop) the
+//               copy of the final B into the state.
 //               Follows _evolve_gas_state_unsplit_inner / _evolve_gas_state_unsplit
 //               (astronomix/_finite_volume/_state_evolution/evolve_state.py:338-469).
+//   * k_lf_alpha, k_am_switch, k_pp_factors  grid-wide or per-cell inputs of
+//               the Lax-Friedrichs, hybrid / AM-HLLC and VAN_ALBADA_PP stages.
 //   * k_mag     two consecutive fixed-point iterates of the implicit-midpoint
 //               magnetic sub-step with their convergence residuals
 //               (astronomix/_finite_volume/_magnetic_update/_magnetic_field_update.py:82-165).
