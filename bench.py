@@ -458,6 +458,14 @@ def main():
             "k_mag_launches": tm.mag_launches,
             "k_stage_avg_ms": round(stage_avg_ms, 5),
             "k_stage_launches": tm.stage_launches,
+            # the metric's own byte model: the reference dataflow's compulsory
+            # traffic, one pass per RK stage or global sync, 264 + 76 K B per
+            # zone-update in 3D (K fixed-point iterations per step)
+            "metric_model": {
+                "bytes_per_zone_update": round(264.0 + 76.0 * kbar, 1),
+                "effective_GBs": round((264.0 + 76.0 * kbar) * cells * steps / el / 1e9, 1),
+                "frac_of_peak": round((264.0 + 76.0 * kbar) * cells * steps / el / 1e9 / HBM_PEAK_GBS, 4),
+            },
         },
         "stream_copy": None,
         "cpu_baseline": None,
