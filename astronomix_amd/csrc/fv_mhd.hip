@@ -1874,9 +1874,9 @@ amx_status validate(const amx_fv_config* c) {
 
 // G1 (gas rows) | P0 | P1 | P2 (B, v iterates of the fixed point)
 // G1 (gas rows) | P0 | P1 | P2 (B, v iterate sets) | PF (fixed pressure) | P3
-// (the fourth set, single simulations: the loop keeps B in the sets)
-long ws_fields(const amx_fv_config* cfg, int own_state) {
-  return (cfg->dim + 2) + (cfg->mhd ? 19 + (own_state ? 0 : 6) : 0);
+// (the fourth set: step loops and pools keep B in the sets)
+long ws_fields(const amx_fv_config* cfg, int /*own_state*/) {
+  return (cfg->dim + 2) + (cfg->mhd ? 25 : 0);
 }
 long state_fields(const amx_fv_config* cfg) { return (cfg->dim + 2) + (cfg->mhd ? 3 : 0); }
 long cells(const amx_fv_config* cfg) { return (long)cfg->nx * cfg->ny * (cfg->dim == 3 ? cfg->nz : 1); }
@@ -2020,7 +2020,7 @@ struct Bufs {
   float* gas[5];
   float* B[3];
   float* G1;
-  float* P[4][6];  // iterate sets: launch k writes P[iterate_set(bset, k)]; P[3] single simulations only
+  float* P[4][6];  // iterate sets: launch k writes P[iterate_set(bset, k)]
   float* PF;       // energy-fixed pressure of the magnetic sub-step's output (MHD)
 };
 
@@ -2038,7 +2038,7 @@ Bufs bufs(const amx_fv_ctx* c, float* state) {
     for (int j = 0; j < 3; ++j)
       for (int k = 0; k < 6; ++k) b.P[j][k] = ws + (long)(c->NV + 6 * j + k) * n;
   if (c->cfg.mhd) b.PF = ws + (long)(c->NV + 18) * n;
-  if (c->cfg.mhd && !c->own_state)
+  if (c->cfg.mhd)
     for (int k = 0; k < 6; ++k) b.P[3][k] = ws + (long)(c->NV + 19 + k) * n;
   return b;
 }
@@ -2795,6 +2795,9 @@ extern "C" amx_status amx_fv_pool_create(const amx_fv_config* cfg, const amx_fv_
   c->sp.dx = c->dx;
   c->sp.dim = cfg->dim;
   c->use_graphs = env_int("AMX_POOL_GRAPHS", 1) != 0;
+  // pool slots keep B in their iterate sets between steps; amx_fv_pool_read
+  // writes it into the slot's state rows
+  c->keep_b = c->cfg.mhd ? 1 : 0;
   amx_fv_pool* p = new (std::nothrow) amx_fv_pool();
   if (!p) {
     free_ctx(c);
@@ -2857,8 +2860,23 @@ extern "C" amx_status amx_fv_pool_read(amx_fv_pool* p, int slot, float* dst, voi
   if (s != AMX_OK) return s;
   if (!dst) return fail(AMX_ERR_INVALID, "null destination%s");
   amx_fv_ctx* c = p->c;
+  hipStream_t st = (hipStream_t)stream;
+  if (c->keep_b) {
+    // the slot's B lives in its iterate set between steps: into its state rows first
+    const Bufs b = bufs(c, nullptr);
+    const long so = (long)slot * c->sstride;
+    BsetArgs ba;
+    for (int j = 0; j < 4; ++j)
+      for (int k = 0; k < 3; ++k) ba.PB[j][k] = b.P[j][k] + so;
+    for (int k = 0; k < 3; ++k) ba.dst[k] = c->mem + so + (long)(c->NV + k) * c->ncell;
+    ba.n = c->ncell;
+    long blocks = (c->ncell + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_copy_bset, dim3(blocks), dim3(256), 0, st, c->ctl + slot, ba);
+    HIPCHK(hipGetLastError());
+  }
   HIPCHK(hipMemcpyAsync(dst, c->mem + (long)slot * c->sstride, sizeof(float) * (size_t)c->NS * c->ncell,
-                        hipMemcpyDefault, (hipStream_t)stream));
+                        hipMemcpyDefault, st));
   return AMX_OK;
 }
 
@@ -2923,7 +2941,7 @@ extern "C" amx_status amx_fv_pool_step_finish(amx_fv_pool* p, void* stream) {
   amx_status s = replay_or_capture(c, st, &x, x != nullptr, [&]() -> amx_status {
     amx_status r = step_phase3(c, b, span, st, !c->sp.fixed);
     if (r != AMX_OK) return r;
-    hipLaunchKernelGGL(k_step_end, dim3(span), dim3(1), 0, st, c->ctl, 0);
+    hipLaunchKernelGGL(k_step_end, dim3(span), dim3(1), 0, st, c->ctl, c->keep_b);
     HIPCHK(hipGetLastError());
     return AMX_OK;
   });
