@@ -200,7 +200,9 @@ amx_status amx_fv_pool_destroy(amx_fv_pool* pool);
  * slot and starts it at t = 0 with its own gamma and t_end. */
 amx_status amx_fv_pool_load(amx_fv_pool* pool, int slot, const float* state, float gamma, float t_end,
                             void* stream);
-/* Copies a slot's state out (host or device memory). */
+/* Copies a slot's state out (host or device memory). With MHD it first
+ * writes the slot's magnetic field, which stays in the fixed point's iterate
+ * buffers between steps, into the slot's state rows (a kernel on `stream`). */
 amx_status amx_fv_pool_read(amx_fv_pool* pool, int slot, float* dst, void* stream);
 /* Empties a slot. */
 amx_status amx_fv_pool_release(amx_fv_pool* pool, int slot, void* stream);
