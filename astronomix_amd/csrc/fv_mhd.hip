@@ -2632,6 +2632,19 @@ extern "C" amx_status amx_fv_time_integration_snapshots(amx_fv_ctx* c, float* st
 
 namespace {
 
+template <int DIM, int AXI>
+void launch_adj_iface(const amx_fv_ctx* c, dim3 grid, dim3 blk, hipStream_t st, const float* Win, const float* NU,
+                      float* con, adj::AdjGeom g, Phys ph, bool duals) {
+  if (c->cfg.riemann == AMX_HLLC && duals)
+    hipLaunchKernelGGL((adj::k_adj_iface<DIM, AXI, AMX_HLLC, true>), grid, blk, 0, st, Win, NU, con, c->ctl, g, ph);
+  else if (c->cfg.riemann == AMX_HLLC)
+    hipLaunchKernelGGL((adj::k_adj_iface<DIM, AXI, AMX_HLLC, false>), grid, blk, 0, st, Win, NU, con, c->ctl, g, ph);
+  else if (duals)
+    hipLaunchKernelGGL((adj::k_adj_iface<DIM, AXI, AMX_HLL, true>), grid, blk, 0, st, Win, NU, con, c->ctl, g, ph);
+  else
+    hipLaunchKernelGGL((adj::k_adj_iface<DIM, AXI, AMX_HLL, false>), grid, blk, 0, st, Win, NU, con, c->ctl, g, ph);
+}
+
 template <int DIM>
 amx_status adjoint_step_t(amx_fv_ctx* c, const float* w0, const float* lam_out, float* lam_in, hipStream_t st) {
   constexpr int NV = DIM + 2;
@@ -2673,6 +2686,8 @@ amx_status adjoint_step_t(amx_fv_ctx* c, const float* w0, const float* lam_out, 
   }
   HIPCHK((launch_stage<SRC_PLAIN, false, SRC_PLAIN>(c, a2, 1, st)));
   const Phys ph = c->hph[0];  // the interface kernels read dt from ctl[0]
+  // AMX_ADJ_DUAL=1: the flux Jacobians by dual numbers (a cross-check of the reverse sweep)
+  const bool duals = env_int("AMX_ADJ_DUAL", 0) != 0;
   long blocks = (n + 255) / 256;
   if (blocks > 8192) blocks = 8192;
   const dim3 grid(blocks), blk(256);
@@ -2688,22 +2703,12 @@ amx_status adjoint_step_t(amx_fv_ctx* c, const float* w0, const float* lam_out, 
     HIPCHK(hipGetLastError());
     for (int ax = 0; ax < DIM; ++ax) {
       float* con = CON + (long)ax * 4 * NV * n;
-      if (ax == 0) {
-        if (c->cfg.riemann == AMX_HLLC)
-          hipLaunchKernelGGL((adj::k_adj_iface<DIM, 0, AMX_HLLC>), grid, blk, 0, st, Win, NU, con, c->ctl, g, ph);
-        else
-          hipLaunchKernelGGL((adj::k_adj_iface<DIM, 0, AMX_HLL>), grid, blk, 0, st, Win, NU, con, c->ctl, g, ph);
-      } else if (ax == 1) {
-        if (c->cfg.riemann == AMX_HLLC)
-          hipLaunchKernelGGL((adj::k_adj_iface<DIM, 1, AMX_HLLC>), grid, blk, 0, st, Win, NU, con, c->ctl, g, ph);
-        else
-          hipLaunchKernelGGL((adj::k_adj_iface<DIM, 1, AMX_HLL>), grid, blk, 0, st, Win, NU, con, c->ctl, g, ph);
-      } else {
-        if (c->cfg.riemann == AMX_HLLC)
-          hipLaunchKernelGGL((adj::k_adj_iface<DIM, 2, AMX_HLLC>), grid, blk, 0, st, Win, NU, con, c->ctl, g, ph);
-        else
-          hipLaunchKernelGGL((adj::k_adj_iface<DIM, 2, AMX_HLL>), grid, blk, 0, st, Win, NU, con, c->ctl, g, ph);
-      }
+      if (ax == 0)
+        launch_adj_iface<DIM, 0>(c, grid, blk, st, Win, NU, con, g, ph, duals);
+      else if (ax == 1)
+        launch_adj_iface<DIM, 1>(c, grid, blk, st, Win, NU, con, g, ph, duals);
+      else
+        launch_adj_iface<DIM, 2>(c, grid, blk, st, Win, NU, con, g, ph, duals);
       HIPCHK(hipGetLastError());
     }
     hipLaunchKernelGGL(adj::k_adj_gather<DIM>, grid, blk, 0, st, Win, NU, CON, dst, stage == 0 ? 1 : 0, g, ph);
