@@ -652,9 +652,11 @@ __global__ void k_face(const FaceArgs a) {
       if (a.k.final_stage) {
         a.bf[i][c] = a.bf[i][c] + r[i];
       } else {
-        const float bn = (a.k.k1 * a.b0[i][c] + a.k.k3 * a.bc[i][c]) + r[i];
+        // a zero weight skips its buffer (the first stage: k3 = ff = 0)
+        const float bn = a.k.k3 != 0.f ? (a.k.k1 * a.b0[i][c] + a.k.k3 * a.bc[i][c]) + r[i]
+                                       : __fmul_rn(a.k.k1, a.b0[i][c]) + r[i];
         a.bc[i][c] = bn;
-        a.bf[i][c] = a.bf[i][c] + bn * a.k.ff;
+        if (a.k.ff != 0.f) a.bf[i][c] = a.bf[i][c] + bn * a.k.ff;
       }
     }
   }
@@ -687,9 +689,11 @@ __global__ void k_qupd(const QupdArgs a) {
       if (a.k.final_stage) {
         a.qf[v][c] = a.qf[v][c] + rhs;
       } else {
-        const float qn = (a.k.k1 * a.q0[v][c] + a.k.k3 * a.qc[v][c]) + rhs;
+        // a zero weight skips its buffer (the first stage: k3 = ff = 0)
+        const float qn = a.k.k3 != 0.f ? (a.k.k1 * a.q0[v][c] + a.k.k3 * a.qc[v][c]) + rhs
+                                       : __fmul_rn(a.k.k1, a.q0[v][c]) + rhs;
         a.qc[v][c] = qn;
-        a.qf[v][c] = a.qf[v][c] + qn * a.k.ff;
+        if (a.k.ff != 0.f) a.qf[v][c] = a.qf[v][c] + qn * a.k.ff;
       }
     }
   }
