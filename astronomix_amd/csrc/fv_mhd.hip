@@ -76,6 +76,7 @@ struct Ctl {
   unsigned resp[NRES][NSLOT];   // fixed-point residual partials, by launch index % NRES
   unsigned lfp[3][NSLOT];       // Lax-Friedrichs: partial maxima of |u| + c of the stage input
   int bset;                     // iterate set holding B0 of the magnetic sub-step; -1: the state's B rows
+  int held;                     // the fixed point had not settled when k_mag_final ran: gated stage kernels return
 };
 
 // Iterate set of fixed-point launch k: the three sets other than bset
@@ -448,6 +449,7 @@ struct StageArgs {
   long sstride;
   int slot0;
   int want_cfl;          // reduce the CFL maxima of the output into ctl[s].cflp
+  int gated;             // return at once while ctl[s].held is set
   int copy_b;            // copy B_K (selb) into bdst
   int nx, na, nb;        // 3D: (nx, ny, nz); 2D: (1, nx, ny)
   int ntb, nta, ncx, lx;
@@ -503,6 +505,7 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
   const int slot = a.slot0 + blockIdx.y;
   Ctl* const C = a.ctl + slot;
   if (!C->run) return;  // block-uniform: empty or finished slot
+  if (a.gated && C->held) return;  // queued behind a fixed point that needs more launches
   const long so = slot * a.sstride;
   const int par = iterate_set(C->bset, C->iters);  // set of the fixed point's final iterate
   const float* src[8];
@@ -1494,15 +1497,23 @@ __global__ __launch_bounds__(((TA + 4) * (TBM + 4) + 255) / 256 * 256) void k_ma
 
 // Final launch index of the fixed point of each running slot (one thread per
 // slot) from its last executed pair, recorded in ctl->iters: the stage
-// kernels after the sub-step read the iterate of that launch.
-__global__ void k_mag_final(Ctl* ctl, float tol, int max_iter) {
+// kernels after the sub-step read the iterate of that launch. spec: the call
+// is queued before the host has seen the residuals; if the reference's loop
+// would still run, it sets held (the gated stage kernels behind it return)
+// and leaves iters for the call after the further launches.
+__global__ void k_mag_final(Ctl* ctl, float tol, int max_iter, int spec) {
   Ctl* const c = ctl + blockIdx.x;
   if (!c->run) return;
   const int L = c->iters - 1;
   const float r0 = __uint_as_float(slots_max(c->resp[L % NRES]));
   const float r1 = __uint_as_float(slots_max(c->resp[(L + 1) % NRES]));
   const int f = pair_final(L, r0, r1, tol, max_iter);
+  if (f < 0 && spec) {
+    c->held = 1;
+    return;
+  }
   c->iters = f >= 0 ? f : L + 1;
+  c->held = 0;
 }
 
 // Unsplit CFL maxima max(|u_a| + c) per axis (_timestep_estimator.py:118-126)
@@ -1676,6 +1687,7 @@ __global__ void k_slot_reset(Ctl* ctl, int slot0, float t_end, int live) {
   c->run = live;
   c->t_end = t_end;
   c->bset = -1;
+  c->held = 0;
 }
 
 // Streaming copy, 4 floats per lane per iteration.
@@ -1779,8 +1791,8 @@ struct amx_fv_ctx {
     int e0, e1, cls, k;
   };
   std::vector<hipEvent_t> ev;
+  std::vector<int> ev_free;  // indices into ev of events no record holds
   std::vector<Rec> pending;
-  int ev_used;
   double ms[2];
   long launches[2];
 };
@@ -1795,38 +1807,61 @@ enum { CLS_STAGE = 0, CLS_MAG = 1 };
 
 int time_begin(amx_fv_ctx* c, hipStream_t st) {
   if (!c->timing) return -1;
-  if (c->ev_used + 2 > (int)c->ev.size()) {
+  while (c->ev_free.size() < 2) {
     hipEvent_t e;
-    for (int i = 0; i < 2; ++i) {
-      if (hipEventCreate(&e) != hipSuccess) return -1;
-      c->ev.push_back(e);
-    }
+    if (hipEventCreate(&e) != hipSuccess) return -1;
+    c->ev_free.push_back((int)c->ev.size());
+    c->ev.push_back(e);
   }
-  const int i = c->ev_used;
-  c->ev_used += 2;
+  const int i = c->ev_free.back();
+  c->ev_free.pop_back();
   (void)hipEventRecord(c->ev[i], st);
   return i;
 }
 
 void time_end(amx_fv_ctx* c, hipStream_t st, int i, int cls, int k) {
   if (i < 0) return;
-  (void)hipEventRecord(c->ev[i + 1], st);
-  c->pending.push_back({i, i + 1, cls, k});
+  const int j = c->ev_free.back();  // time_begin left at least one
+  c->ev_free.pop_back();
+  (void)hipEventRecord(c->ev[j], st);
+  c->pending.push_back({i, j, cls, k});
 }
 
-// Call only after the stream is synchronised: folds completed records into the
-// totals; fixed-point launches with k > K exited early and are dropped.
-void time_resolve(amx_fv_ctx* c, int K) {
-  for (const auto& r : c->pending) {
-    if (r.cls == CLS_MAG && r.k > K) continue;
+// Folds the first n pending records, whose events have completed, into the
+// totals and frees their events.
+void time_resolve_first(amx_fv_ctx* c, size_t n) {
+  if (n > c->pending.size()) n = c->pending.size();
+  for (size_t i = 0; i < n; ++i) {
+    const auto& r = c->pending[i];
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, c->ev[r.e0], c->ev[r.e1]) == hipSuccess) {
       c->ms[r.cls] += ms;
       c->launches[r.cls] += 1;
     }
+    c->ev_free.push_back(r.e0);
+    c->ev_free.push_back(r.e1);
   }
-  c->pending.clear();
-  c->ev_used = 0;
+  c->pending.erase(c->pending.begin(), c->pending.begin() + n);
+}
+
+// Call only after the stream is synchronised.
+void time_resolve(amx_fv_ctx* c) { time_resolve_first(c, c->pending.size()); }
+
+// Drops the pending records from index `from` on that `drop` selects (launches
+// that exited early or were held), freeing their events.
+template <class F>
+void time_drop(amx_fv_ctx* c, size_t from, F&& drop) {
+  size_t w = from;
+  for (size_t i = from; i < c->pending.size(); ++i) {
+    const auto r = c->pending[i];
+    if (drop(r)) {
+      c->ev_free.push_back(r.e0);
+      c->ev_free.push_back(r.e1);
+    } else {
+      c->pending[w++] = r;
+    }
+  }
+  c->pending.resize(w);
 }
 
 int env_int(const char* name, int dflt) {
@@ -2375,11 +2410,11 @@ amx_status fixed_point_more(amx_fv_ctx* c, const Bufs& b, int nspan, hipStream_t
 
 // Phase 3: S_A(dt/2) on the magnetically updated gas (v <- v_K, energy fix on
 // the fly); the final iterate of every slot is picked on the device.
-amx_status step_phase3(amx_fv_ctx* c, const Bufs& b, int nspan, hipStream_t st, int want_cfl) {
+amx_status step_phase3(amx_fv_ctx* c, const Bufs& b, int nspan, hipStream_t st, int want_cfl, int spec) {
   if (!c->cfg.mhd) return AMX_OK;
   const int NV = c->NV;
   hipLaunchKernelGGL(k_mag_final, dim3(nspan), dim3(1), 0, st, c->ctl, c->cfg.fixed_point_tol,
-                     c->cfg.max_fixed_point_iterations);
+                     c->cfg.max_fixed_point_iterations, spec);
   HIPCHK(hipGetLastError());
   StageArgs b1 = stage_base(c, 0.5f);
   for (int v = 0; v < NV; ++v) {
@@ -2387,6 +2422,7 @@ amx_status step_phase3(amx_fv_ctx* c, const Bufs& b, int nspan, hipStream_t st, 
     b1.out[v] = b.G1 + (long)v * c->ncell;
   }
   b1.pfix = b.PF;
+  b1.gated = spec;
   for (int j = 0; j < 4; ++j)
     for (int k = 0; k < 3; ++k) {
       b1.selv[j][k] = b.P[j][3 + k];
@@ -2394,7 +2430,7 @@ amx_status step_phase3(amx_fv_ctx* c, const Bufs& b, int nspan, hipStream_t st, 
     }
   int e = time_begin(c, st);
   HIPCHK((launch_stage<SRC_MAGFIX, false, SRC_PLAIN>(c, b1, nspan, st)));
-  time_end(c, st, e, CLS_STAGE, 0);
+  time_end(c, st, e, CLS_STAGE, spec);
   StageArgs b2 = stage_base(c, 0.5f);
   for (int v = 0; v < NV; ++v) {
     b2.src[v] = b.G1 + (long)v * c->ncell;
@@ -2411,9 +2447,10 @@ amx_status step_phase3(amx_fv_ctx* c, const Bufs& b, int nspan, hipStream_t st, 
   b2.copy_b = c->keep_b ? 0 : 1;
   b2.want_cfl = want_cfl;
   b2.pfix = b.PF;
+  b2.gated = spec;
   e = time_begin(c, st);
   HIPCHK((launch_stage<SRC_PLAIN, true, SRC_PFIX>(c, b2, nspan, st)));
-  time_end(c, st, e, CLS_STAGE, 0);
+  time_end(c, st, e, CLS_STAGE, spec);
   return AMX_OK;
 }
 
@@ -2422,20 +2459,34 @@ amx_status step_phase3(amx_fv_ctx* c, const Bufs& b, int nspan, hipStream_t st, 
 amx_status do_step(amx_fv_ctx* c, float* state, hipStream_t st, int want_cfl, int* k_out) {
   const Bufs b = bufs(c, state);
   int klast = 0;
+  size_t rec0 = c->pending.size();  // this step's timing records start here
   amx_status s = step_phase1(c, b, 1, st, want_cfl, &klast);
   if (s != AMX_OK) return s;
   int K = 0;
   if (c->cfg.mhd) {
     HIPCHK(hipMemcpyAsync(c->hctl, c->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    s = fixed_point_more(c, b, 1, st, &klast);
+    HIPCHK(hipEventRecord(c->ev_status, st));
+    // The second half is queued before the residuals reach the host, so the
+    // GPU does not wait for it; its stage kernels run only if the launched
+    // pairs settled the fixed point (k_mag_final sets held otherwise).
+    s = step_phase3(c, b, 1, st, want_cfl, 1);
     if (s != AMX_OK) return s;
-    fixed_point_settled(c, 1, &K);
-    time_resolve(c, K);
+    HIPCHK(hipEventSynchronize(c->ev_status));
+    // earlier steps' work has completed
+    time_resolve_first(c, rec0);
+    rec0 = 0;
+    if (!fixed_point_settled(c, 1, &K)) {
+      time_drop(c, rec0, [](const amx_fv_ctx::Rec& r) { return r.cls == CLS_STAGE && r.k == 1; });
+      s = fixed_point_more(c, b, 1, st, &klast);
+      if (s != AMX_OK) return s;
+      fixed_point_settled(c, 1, &K);
+      s = step_phase3(c, b, 1, st, want_cfl, 0);
+      if (s != AMX_OK) return s;
+    }
+    // launches past the final one exited early
+    time_drop(c, rec0, [K](const amx_fv_ctx::Rec& r) { return r.cls == CLS_MAG && r.k > K; });
     c->last_k = K;
   }
-  s = step_phase3(c, b, 1, st, want_cfl);
-  if (s != AMX_OK) return s;
   if (k_out) *k_out = K;
   return AMX_OK;
 }
@@ -2464,8 +2515,7 @@ extern "C" amx_status amx_fv_destroy(amx_fv_ctx* c) {
 extern "C" amx_status amx_fv_set_timing(amx_fv_ctx* c, int enable) {
   if (!c) return fail(AMX_ERR_INVALID, "null context%s");
   HIPCHK(hipDeviceSynchronize());
-  c->pending.clear();
-  c->ev_used = 0;
+  time_drop(c, 0, [](const amx_fv_ctx::Rec&) { return true; });
   c->timing = enable ? 1 : 0;
   c->ms[0] = c->ms[1] = 0.0;
   c->launches[0] = c->launches[1] = 0;
@@ -2475,7 +2525,7 @@ extern "C" amx_status amx_fv_set_timing(amx_fv_ctx* c, int enable) {
 extern "C" amx_status amx_fv_get_timing(amx_fv_ctx* c, amx_fv_timing* out) {
   if (!c || !out) return fail(AMX_ERR_INVALID, "null argument%s");
   HIPCHK(hipDeviceSynchronize());
-  time_resolve(c, 1 << 30);
+  time_resolve(c);
   out->stage_ms = c->ms[CLS_STAGE];
   out->stage_launches = c->launches[CLS_STAGE];
   out->mag_ms = c->ms[CLS_MAG];
@@ -2604,7 +2654,7 @@ amx_status fv_loop(amx_fv_ctx* c, float* state, const amx_fv_run* run, snap::Dri
       // hydro steps have no fixed-point readback: read dt here
       HIPCHK(hipMemcpyAsync(c->hctl, c->ctl, offsetof(Ctl, cflp), hipMemcpyDeviceToHost, st));
       HIPCHK(hipStreamSynchronize(st));
-      time_resolve(c, 1 << 30);
+      time_resolve(c);
       t = c->hctl[0].t;
     } else {
       // hctl holds dt and t from the fixed-point readback (before k_step_end)
@@ -2633,7 +2683,7 @@ amx_status fv_loop(amx_fv_ctx* c, float* state, const amx_fv_run* run, snap::Dri
   }
   HIPCHK(hipMemcpyAsync(c->hctl, c->ctl, offsetof(Ctl, cflp), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
-  time_resolve(c, 1 << 30);
+  time_resolve(c);
   if (snp) snp->finish(steps);
   if (stats) {
     stats->steps = steps;
@@ -2996,7 +3046,7 @@ extern "C" amx_status amx_fv_pool_step_finish(amx_fv_pool* p, void* stream) {
   auto it = c->g_finish.find(span);
   hipGraphExec_t x = it == c->g_finish.end() ? nullptr : it->second;
   amx_status s = replay_or_capture(c, st, &x, x != nullptr, [&]() -> amx_status {
-    amx_status r = step_phase3(c, b, span, st, !c->sp.fixed);
+    amx_status r = step_phase3(c, b, span, st, !c->sp.fixed, 0);
     if (r != AMX_OK) return r;
     hipLaunchKernelGGL(k_step_end, dim3(span), dim3(1), 0, st, c->ctl, c->keep_b);
     HIPCHK(hipGetLastError());

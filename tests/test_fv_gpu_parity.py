@@ -249,3 +249,38 @@ def test_loop_keeping_b_in_the_iterate_sets_equals_stepwise_copies(gpu, case):
     ctx.time_integration(s_again, run)
     torch.cuda.synchronize()
     assert torch.equal(s_again, s_loop)
+
+
+def test_fixed_point_past_the_queued_launches(gpu, oracle):
+    """At fixed-point tolerance 1e-6 the first step of a fresh context ends at
+    launch 6, past the launches (0..5) it queues before the host has read the
+    residuals. The second half of the step, queued behind them, is held on the
+    device and runs after the further launches. The loop and single steps
+    both take that route and equal each other bit for bit, and the oracle to
+    the bar with its iteration counts."""
+    import torch
+    print("[gpu] fixed point past the queued launches", flush=True)
+    state = W.random_smooth((24, 24, 24), True, seed=1, amp=1.0)
+    cfg = W.fv_config(3, 24, mhd=True, riemann=HLL)
+    nc = native_config(state.shape, cfg, 5.0 / 3.0, 0.8)
+    nc.fixed_point_tol = 1e-6
+    steps, t_end = 12, 0.02
+    loop = _native.FvContext(_native.AmxFvConfig.from_buffer_copy(nc))
+    s_loop = torch.from_numpy(np.ascontiguousarray(state)).cuda()
+    # no end-time clamp: every step of the loop takes the same dt as the single steps
+    run = _native.AmxFvRun(t_end=t_end, fixed_timestep=1, num_timesteps=steps, exact_end_time=0, max_steps=-1)
+    st = loop.time_integration(s_loop, run)
+    single = _native.FvContext(_native.AmxFvConfig.from_buffer_copy(nc))
+    s_step = torch.from_numpy(np.ascontiguousarray(state)).cuda()
+    dt = torch.tensor(np.float32(t_end) / np.float32(steps), dtype=torch.float32, device="cuda")
+    ks = [single.evolve_state(s_step, dt) for _ in range(steps)]
+    torch.cuda.synchronize()
+    assert ks[0] > 5, ks
+    assert st.steps == steps and st.fixed_point_iterations == sum(ks)
+    assert torch.equal(s_loop, s_step)
+    oc = oracle.make_config(state.shape, 3, mhd=True, C_cfl=0.8, t_end=t_end, tol=1e-6, fixed_timestep=True,
+                            num_timesteps=steps, exact_end_time=False)
+    ref, osteps, _, iters = oracle.time_integration(state.astype(np.float32), oc)
+    assert osteps == steps and iters == sum(ks)
+    err = rel_l1(s_loop.cpu().numpy(), ref)
+    assert err.max() < TOL, err
