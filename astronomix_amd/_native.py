@@ -281,8 +281,9 @@ class FvContext:
                                            self._stream(stream)), "amx_fv_evolve_state")
         return int(k.value)
 
-    def set_timing(self, enable: bool):
-        check(self.lib.amx_fv_set_timing(self.handle, int(bool(enable))), "amx_fv_set_timing")
+    def set_timing(self, every: int):
+        """0: off; n >= 1: HIP-event timing of the kernels of every n-th step."""
+        check(self.lib.amx_fv_set_timing(self.handle, max(0, int(every))), "amx_fv_set_timing")
 
     def get_timing(self) -> AmxFvTiming:
         t = AmxFvTiming()

@@ -1785,8 +1785,10 @@ struct amx_fv_ctx {
   std::vector<int> live;
   std::vector<int> fresh;  // loaded since the last status readback
   std::vector<long> iters_total;
-  // optional HIP-event timing of kernel launches (single simulation)
+  // optional HIP-event timing of kernel launches (single simulation) on
+  // every timing-th step; tstep counts steps since it was set
   int timing;
+  long tstep;
   struct Rec {
     int e0, e1, cls, k;
   };
@@ -1806,7 +1808,7 @@ namespace {
 enum { CLS_STAGE = 0, CLS_MAG = 1 };
 
 int time_begin(amx_fv_ctx* c, hipStream_t st) {
-  if (!c->timing) return -1;
+  if (c->timing <= 0 || c->tstep % c->timing != 0) return -1;
   while (c->ev_free.size() < 2) {
     hipEvent_t e;
     if (hipEventCreate(&e) != hipSuccess) return -1;
@@ -2487,6 +2489,7 @@ amx_status do_step(amx_fv_ctx* c, float* state, hipStream_t st, int want_cfl, in
     time_drop(c, rec0, [K](const amx_fv_ctx::Rec& r) { return r.cls == CLS_MAG && r.k > K; });
     c->last_k = K;
   }
+  ++c->tstep;
   if (k_out) *k_out = K;
   return AMX_OK;
 }
@@ -2516,7 +2519,8 @@ extern "C" amx_status amx_fv_set_timing(amx_fv_ctx* c, int enable) {
   if (!c) return fail(AMX_ERR_INVALID, "null context%s");
   HIPCHK(hipDeviceSynchronize());
   time_drop(c, 0, [](const amx_fv_ctx::Rec&) { return true; });
-  c->timing = enable ? 1 : 0;
+  c->timing = enable > 0 ? enable : 0;
+  c->tstep = 0;
   c->ms[0] = c->ms[1] = 0.0;
   c->launches[0] = c->launches[1] = 0;
   return AMX_OK;

@@ -353,6 +353,8 @@ def main():
     ap.add_argument("--hbm-budget-gib", type=float, default=32.0, help="serving_mix: pool memory budget")
     ap.add_argument("--no-kernel-timing", action="store_true",
                     help="time the steps without per-launch HIP events (no roofline figures)")
+    ap.add_argument("--timing-every", type=int, default=4,
+                    help="mhd_blast_3d: HIP events around the launches of every n-th timed step")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -380,7 +382,8 @@ def main():
     if args.warmup > 0:
         ctx.time_integration(state, run_w)
     torch.cuda.synchronize()
-    ctx.set_timing(not args.no_kernel_timing)
+    every = 0 if args.no_kernel_timing else max(1, args.timing_every)
+    ctx.set_timing(every)
     run = _native.AmxFvRun(t_end=float(params.t_end), fixed_timestep=0, num_timesteps=0, exact_end_time=1,
                            max_steps=args.steps)
     torch.cuda.synchronize()
@@ -389,7 +392,7 @@ def main():
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     tm = ctx.get_timing()
-    ctx.set_timing(False)
+    ctx.set_timing(0)
     if not bool(torch.isfinite(state).all()):
         print("non-finite state after the timed steps", file=sys.stderr)
         sys.exit(1)
@@ -400,7 +403,8 @@ def main():
     kbar = st.fixed_point_iterations / max(steps, 1)
     # executed pair launches per step (from the event records; without them,
     # the mean of floor(K / 2) + 1 is approximated by K / 2 + 1)
-    pairs = tm.mag_launches / max(steps, 1) if tm.mag_launches else kbar / 2.0 + 1.0
+    sampled = -(-steps // every) if every else 0  # steps whose launches carry events
+    pairs = tm.mag_launches / max(sampled, 1) if tm.mag_launches else kbar / 2.0 + 1.0
     mag_fields = MAG_FIELDS_FIRST + MAG_FIELDS_LATER * max(pairs - 1.0, 0.0)
     step_bytes = (STEP_FIELDS_STAGES + mag_fields) * 4.0
     MAG_BYTES_PER_CELL = mag_fields * 4.0 / max(pairs, 1.0)
@@ -447,6 +451,8 @@ def main():
             "traffic": traffic,
             "bytes_per_cell_per_launch": per_cell,
             "avg_launch_ms": round(avg_ms, 5),
+            "launch_timing": (f"HIP events around each launch of one timed step in {every} "
+                              f"({sampled} of {steps} steps)" if every else None),
         },
         "step_model": {
             "fixed_point_iterations_per_step": round(kbar, 3),

@@ -162,7 +162,8 @@ typedef struct amx_fv_timing {
   long mag_launches;
 } amx_fv_timing;
 
-/* Enables (1) / disables (0) event timing on ctx and resets its counters.
+/* Event timing on ctx: 0 off, 1 every step, n > 1 every n-th step (the
+ * events themselves cost time on the stream); resets the counters.
  * No counterpart in the reference (its timing is wall-clock around the jitted
  * loop, time_integration.py:194-234); used by the benchmark harness. */
 amx_status amx_fv_set_timing(amx_fv_ctx* ctx, int enable);
