@@ -2632,6 +2632,11 @@ amx_status fv_loop(amx_fv_ctx* c, float* state, const amx_fv_run* run, snap::Dri
   long steps = 0, iters = 0;
   int kmax = 0;
   float t = 0.f;
+  // Hydro steps without snapshot rows or a hook need nothing from the host
+  // between steps: they are queued in batches and t is read once per batch
+  // (k_step_begin clears ctl.run past the end, so surplus steps do nothing).
+  const bool batch = !c->cfg.mhd && !snp && !c->hook;
+  constexpr long BATCH = 16;
   for (;;) {
     if (run->fixed_timestep) {
       if (steps >= run->num_timesteps) break;
@@ -2639,6 +2644,25 @@ amx_status fv_loop(amx_fv_ctx* c, float* state, const amx_fv_run* run, snap::Dri
       break;
     }
     if (run->max_steps >= 0 && steps >= run->max_steps) break;
+    if (batch) {
+      long n = BATCH;
+      if (run->max_steps >= 0 && run->max_steps - steps < n) n = run->max_steps - steps;
+      if (run->fixed_timestep && run->num_timesteps - steps < n) n = run->num_timesteps - steps;
+      for (long i = 0; i < n; ++i) {
+        hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(1), 0, st, c->ctl, sp);
+        HIPCHK(hipGetLastError());
+        amx_status s = do_step(c, state, st, !run->fixed_timestep, nullptr);
+        if (s != AMX_OK) return s;
+        hipLaunchKernelGGL(k_step_end, dim3(1), dim3(1), 0, st, c->ctl, 0);
+        HIPCHK(hipGetLastError());
+      }
+      HIPCHK(hipMemcpyAsync(c->hctl, c->ctl, offsetof(Ctl, cflp), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      time_resolve(c);
+      t = c->hctl[0].t;
+      steps = c->hctl[0].steps;
+      continue;
+    }
     if (snp) {
       amx_status s = snp->begin(t, state, st);
       if (s != AMX_OK) return s;

@@ -284,3 +284,34 @@ def test_fixed_point_past_the_queued_launches(gpu, oracle):
     assert osteps == steps and iters == sum(ks)
     err = rel_l1(s_loop.cpu().numpy(), ref)
     assert err.max() < TOL, err
+
+
+@pytest.mark.parametrize("max_steps", [-1, 3, 21])
+def test_hydro_loop_batches_equal_single_steps(gpu, max_steps):
+    """Hydro loops without snapshot rows or a step hook queue their steps in
+    batches of 16 and read t once per batch; steps queued past the end do
+    nothing. A fixed-dt loop of 21 steps (one full batch and part of one),
+    alone or cut at 3 or 21 steps by max_steps, equals single steps bit for
+    bit and reports the steps it took."""
+    import torch
+    print(f"[gpu] hydro loop batches, max_steps {max_steps}", flush=True)
+    state = W.kelvin_helmholtz_2d(64)
+    cfg = W.fv_config(2, 64, mhd=False, riemann=HLLC)
+    ctx = _native.fv_context(native_config(state.shape, cfg, 5.0 / 3.0, 0.4))
+    steps, t_end = 21, 0.021
+    run = _native.AmxFvRun(t_end=t_end, fixed_timestep=1, num_timesteps=steps, exact_end_time=0,
+                           max_steps=max_steps)
+    s_loop = torch.from_numpy(np.ascontiguousarray(state)).cuda()
+    st = ctx.time_integration(s_loop, run)
+    n = steps if max_steps < 0 else min(steps, max_steps)
+    assert st.steps == n
+    s_step = torch.from_numpy(np.ascontiguousarray(state)).cuda()
+    dt = torch.tensor(np.float32(t_end) / np.float32(steps), dtype=torch.float32, device="cuda")
+    for _ in range(n):
+        ctx.evolve_state(s_step, dt)
+    torch.cuda.synchronize()
+    assert torch.equal(s_loop, s_step)
+    t = np.float32(0.0)
+    for _ in range(n):
+        t = np.float32(t + np.float32(t_end) / np.float32(steps))
+    assert st.t == float(t)
