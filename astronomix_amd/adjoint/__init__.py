@@ -87,9 +87,12 @@ def step_sizes(state, ctx, t_end: float, num_steps: int = None, exact_end: bool 
     dts, t = [], torch.zeros((), dtype=torch.float32, device=state.device)
     d = torch.empty((), dtype=torch.float32, device=state.device)
     t_end_t = torch.tensor(float(t_end), dtype=torch.float32, device=state.device)
-    while float(t) < t_end:
+    # the loop condition t < t_end is evaluated in float32, as the device loop
+    # does (time_integration.py:772-777): comparing a float32 t with the double
+    # t_end never ends once the clamp has put t on float32(t_end) < t_end
+    while bool(t < t_end_t):
         ctx.cfl_dt(s, d)
-        dt = torch.minimum(d, t_end_t - t).clone()
+        dt = torch.minimum(d, t_end_t - t).clone() if exact_end else d.clone()
         ctx.evolve_state(s, dt)
         t = t + dt
         dts.append(dt)

@@ -148,8 +148,9 @@ def random_smooth(shape, mhd: bool, seed: int = 0, amp: float = 0.2, dtype=np.fl
         s[i] = 0.5 * field()
     s[dim + 1] = 1.0 + amp * np.tanh(field())
     if mhd:
-        # divergence-free B from a vector potential (central differences keep
-        # div B = 0 only approximately; the fixed point preserves whatever div B is)
+        # B is random smooth modes, not divergence-free: the magnetic fixed
+        # point is a discrete curl update, so it keeps whatever central div B
+        # the state starts with, which is all the parity and invariant tests need
         for i in range(3):
             s[dim + 2 + i] = 0.3 * field()
     return s.astype(dtype)

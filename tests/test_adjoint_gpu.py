@@ -199,3 +199,29 @@ def test_reverse_sweep_equals_dual_number_jacobians(gpu, shape, riemann, seed, m
     assert np.all(np.isfinite(rev))
     scale = np.abs(dual).max()
     assert np.abs(rev - dual).max() <= 2e-5 * scale, np.abs(rev - dual).max() / scale
+
+
+@pytest.mark.parametrize("t_end", [0.07, 0.7])
+def test_adaptive_rollout_steps_equal_the_device_loop(gpu, t_end):
+    """Adaptive dt (no fixed_timestep): the rollout's recorded steps are the
+    device loop's steps -- same count, bit-identical final state -- also for
+    t_end = 0.7, whose float32 value lies below the double (the host loop
+    compares t with t_end in float32, as the device does)."""
+    import torch
+    from astronomix_amd.adjoint import rollout
+    from astronomix_amd.option_classes.simulation_params import SimulationParams
+    print(f"[gpu] adaptive rollout vs device loop, t_end {t_end}", flush=True)
+    s = W.kelvin_helmholtz_2d(32, seed=2)
+    cfg = W.fv_config(2, 32, mhd=False, riemann=1)
+    params = SimulationParams(C_cfl=0.4, t_end=t_end, gamma=5 / 3)
+    ctx = _native.fv_context(native_config(s.shape, cfg, params.gamma, params.C_cfl))
+    S = torch.from_numpy(s).cuda()
+    st = ctx.time_integration(S, _native.AmxFvRun(t_end=t_end, fixed_timestep=0, num_timesteps=0,
+                                                  exact_end_time=1, max_steps=-1))
+    w = torch.from_numpy(s).cuda().requires_grad_(True)
+    out = rollout(w, cfg, params, checkpoint_every=50)
+    torch.cuda.synchronize()
+    from astronomix_amd.adjoint import step_sizes
+    n = len(step_sizes(torch.from_numpy(s).cuda(), ctx, t_end))
+    assert n == st.steps
+    np.testing.assert_array_equal(out.detach().cpu().numpy(), S.cpu().numpy())
