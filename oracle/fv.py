@@ -54,38 +54,48 @@ def build() -> str:
     return _LIB_PATH
 
 
+def use_library(path=None):
+    """Bind the functions below to the restatement built at `path` (None: the
+    default build). The mutation tests bind copies with planted mistakes."""
+    global _lib
+    _lib = None if path is None else _bind(ctypes.CDLL(path))
+
+
 def _load():
     global _lib
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
             build()
-        lib = ctypes.CDLL(_LIB_PATH)
-        P = ctypes.c_void_p
-        for sfx in ("f32", "f64"):
-            getattr(lib, f"oracle_time_integration_{sfx}").argtypes = [
-                ctypes.POINTER(OracleConfig), P, ctypes.c_long,
-                ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_long)]
-            getattr(lib, f"oracle_time_integration_{sfx}").restype = ctypes.c_long
-            getattr(lib, f"oracle_evolve_state_fv_{sfx}").argtypes = [
-                ctypes.POINTER(OracleConfig), P, ctypes.c_double]
-            getattr(lib, f"oracle_evolve_state_fv_{sfx}").restype = ctypes.c_int
-            getattr(lib, f"oracle_cfl_dt_{sfx}").argtypes = [ctypes.POINTER(OracleConfig), P]
-            getattr(lib, f"oracle_cfl_dt_{sfx}").restype = ctypes.c_double
-            getattr(lib, f"oracle_gas_stage_{sfx}").argtypes = [
-                ctypes.POINTER(OracleConfig), P, P, ctypes.c_double]
-            getattr(lib, f"oracle_gas_rk2_{sfx}").argtypes = [
-                ctypes.POINTER(OracleConfig), P, ctypes.c_double]
-            getattr(lib, f"oracle_magnetic_update_{sfx}").argtypes = [
-                ctypes.POINTER(OracleConfig), P, P, ctypes.c_double]
-            getattr(lib, f"oracle_magnetic_update_{sfx}").restype = ctypes.c_int
-            getattr(lib, f"oracle_riemann_{sfx}").argtypes = [
-                ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_long, P, P, P]
-            getattr(lib, f"oracle_minmod_{sfx}").argtypes = [ctypes.c_long, P, P, P]
-            getattr(lib, f"oracle_limit_{sfx}").argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_long, P, P, P]
-            getattr(lib, f"oracle_am_switch_{sfx}").argtypes = [ctypes.POINTER(OracleConfig), P, P]
-            getattr(lib, f"oracle_pp_factors_{sfx}").argtypes = [ctypes.POINTER(OracleConfig), P, P]
-        _lib = lib
+        _lib = _bind(ctypes.CDLL(_LIB_PATH))
     return _lib
+
+
+def _bind(lib):
+    P = ctypes.c_void_p
+    for sfx in ("f32", "f64"):
+        getattr(lib, f"oracle_time_integration_{sfx}").argtypes = [
+            ctypes.POINTER(OracleConfig), P, ctypes.c_long,
+            ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_long)]
+        getattr(lib, f"oracle_time_integration_{sfx}").restype = ctypes.c_long
+        getattr(lib, f"oracle_evolve_state_fv_{sfx}").argtypes = [
+            ctypes.POINTER(OracleConfig), P, ctypes.c_double]
+        getattr(lib, f"oracle_evolve_state_fv_{sfx}").restype = ctypes.c_int
+        getattr(lib, f"oracle_cfl_dt_{sfx}").argtypes = [ctypes.POINTER(OracleConfig), P]
+        getattr(lib, f"oracle_cfl_dt_{sfx}").restype = ctypes.c_double
+        getattr(lib, f"oracle_gas_stage_{sfx}").argtypes = [
+            ctypes.POINTER(OracleConfig), P, P, ctypes.c_double]
+        getattr(lib, f"oracle_gas_rk2_{sfx}").argtypes = [
+            ctypes.POINTER(OracleConfig), P, ctypes.c_double]
+        getattr(lib, f"oracle_magnetic_update_{sfx}").argtypes = [
+            ctypes.POINTER(OracleConfig), P, P, ctypes.c_double]
+        getattr(lib, f"oracle_magnetic_update_{sfx}").restype = ctypes.c_int
+        getattr(lib, f"oracle_riemann_{sfx}").argtypes = [
+            ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_long, P, P, P]
+        getattr(lib, f"oracle_minmod_{sfx}").argtypes = [ctypes.c_long, P, P, P]
+        getattr(lib, f"oracle_limit_{sfx}").argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_long, P, P, P]
+        getattr(lib, f"oracle_am_switch_{sfx}").argtypes = [ctypes.POINTER(OracleConfig), P, P]
+        getattr(lib, f"oracle_pp_factors_{sfx}").argtypes = [ctypes.POINTER(OracleConfig), P, P]
+    return lib
 
 
 def _sfx(dtype) -> str:
