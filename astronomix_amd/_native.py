@@ -155,6 +155,8 @@ SIGNATURES = {
     "amx_fv_workspace_bytes": (ctypes.c_int, [ctypes.POINTER(AmxFvConfig), ctypes.POINTER(ctypes.c_size_t)]),
     "amx_fv_cfl_dt": (ctypes.c_int, [_P, _P, _P, _P]),
     "amx_fv_evolve_state": (ctypes.c_int, [_P, _P, _P, ctypes.POINTER(ctypes.c_int), _P]),
+    "amx_fv_evolve_state_noalloc": (ctypes.c_int, [ctypes.POINTER(AmxFvConfig), _P, _P,
+                                                   ctypes.POINTER(ctypes.c_int), _P]),
     "amx_fv_time_integration": (ctypes.c_int, [_P, _P, ctypes.POINTER(AmxFvRun), ctypes.POINTER(AmxFvStats), _P]),
     "amx_fv_set_timing": (ctypes.c_int, [_P, ctypes.c_int]),
     "amx_fv_get_timing": (ctypes.c_int, [_P, ctypes.POINTER(AmxFvTiming)]),

@@ -37,6 +37,7 @@
 #include <cstdint>
 #include <functional>
 #include <map>
+#include <mutex>
 #include <new>
 #include <utility>
 #include <vector>
@@ -1962,9 +1963,11 @@ int pick_mag_chunk(long blocks_per_chunk, int nx) {
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
     cus = 256;
   if (nx <= 8) return nx;
+  if (cus > 1024) cus = 1024;
   int best = 8, prev = -1;
   double best_t = 0.0;
-  std::vector<double> fin;
+  double fin_buf[1024];  // the CUs' finish times (no heap use: the allocation-free entry point calls this)
+  double* const fin = fin_buf;
   for (int want = 1; want <= (nx + 7) / 8; ++want) {
     const int lx = (nx + want - 1) / want;
     if (lx < 8 || lx == prev) continue;
@@ -1976,16 +1979,16 @@ int pick_mag_chunk(long blocks_per_chunk, int nx) {
       t = (double)blocks_per_chunk * (nx + 4.0 * ncx) / cus;
     } else {
       // min-heap of the CUs' finish times; blocks dispatch in id order (x chunk slowest)
-      fin.assign(cus, 0.0);
+      std::fill(fin, fin + cus, 0.0);
       for (int k = 0; k < ncx; ++k) {
         const double cost = (k + 1 < ncx ? lx : nx - k * lx) + 4.0;
         for (long j = 0; j < blocks_per_chunk; ++j) {
-          std::pop_heap(fin.begin(), fin.end(), std::greater<double>());
-          fin.back() += cost;
-          std::push_heap(fin.begin(), fin.end(), std::greater<double>());
+          std::pop_heap(fin, fin + cus, std::greater<double>());
+          fin[cus - 1] += cost;
+          std::push_heap(fin, fin + cus, std::greater<double>());
         }
       }
-      t = *std::max_element(fin.begin(), fin.end());
+      t = *std::max_element(fin, fin + cus);
     }
     if (best_t == 0.0 || t < best_t) {
       best_t = t;
@@ -1995,14 +1998,8 @@ int pick_mag_chunk(long blocks_per_chunk, int nx) {
   return best;
 }
 
-// Geometry, launch shapes and allocations of a context of cap slots.
-amx_status make_ctx(const amx_fv_config* cfg, int cap, int own_state, amx_fv_ctx** out) {
-  amx_status s = validate(cfg);
-  if (s != AMX_OK) return s;
-  if (!out) return fail(AMX_ERR_INVALID, "null output%s");
-  if (cap < 1 || cap > 65535) return fail(AMX_ERR_INVALID, "capacity must be in 1..65535%s");
-  amx_fv_ctx* c = new (std::nothrow) amx_fv_ctx();
-  if (!c) return fail(AMX_ERR_NOMEM, "host allocation failed%s");
+// Geometry and launch shapes of a context of cap slots (no allocation).
+void init_geometry(amx_fv_ctx* c, const amx_fv_config* cfg, int cap, int own_state) {
   c->cfg = *cfg;
   c->NV = cfg->dim + 2;
   c->NS = (int)state_fields(cfg);
@@ -2060,6 +2057,19 @@ amx_status make_ctx(const amx_fv_config* cfg, int cap, int own_state, amx_fv_ctx
   c->own_state = own_state;
   const long per_slot = c->ncell * (ws_fields(cfg, own_state) + (own_state ? c->NS : 0));
   c->sstride = own_state ? per_slot : 0;
+  c->last_k = 4;
+}
+
+// Geometry, launch shapes and allocations of a context of cap slots.
+amx_status make_ctx(const amx_fv_config* cfg, int cap, int own_state, amx_fv_ctx** out) {
+  amx_status s = validate(cfg);
+  if (s != AMX_OK) return s;
+  if (!out) return fail(AMX_ERR_INVALID, "null output%s");
+  if (cap < 1 || cap > 65535) return fail(AMX_ERR_INVALID, "capacity must be in 1..65535%s");
+  amx_fv_ctx* c = new (std::nothrow) amx_fv_ctx();
+  if (!c) return fail(AMX_ERR_NOMEM, "host allocation failed%s");
+  init_geometry(c, cfg, cap, own_state);
+  const long per_slot = c->ncell * (ws_fields(cfg, own_state) + (own_state ? c->NS : 0));
   if (hipMalloc((void**)&c->mem, sizeof(float) * (size_t)per_slot * cap) != hipSuccess) {
     c->mem = nullptr;
     free_ctx(c);
@@ -2099,7 +2109,6 @@ amx_status make_ctx(const amx_fv_config* cfg, int cap, int own_state, amx_fv_ctx
   c->live.assign(cap, own_state ? 0 : 1);
   c->fresh.assign(cap, 0);
   c->iters_total.assign(cap, 0);
-  c->last_k = 4;
   *out = c;
   return AMX_OK;
 }
@@ -2579,6 +2588,133 @@ extern "C" amx_status amx_fv_evolve_state(amx_fv_ctx* c, float* state, const flo
   int K = 0;
   amx_status s = do_step(c, state, st, 0, &K);
   if (s != AMX_OK) return s;
+  if (iterations) *iterations = K;
+  return AMX_OK;
+}
+
+// ---- Allocation-free step (amx_fv_evolve_state_noalloc). The workspace,
+// the control block and the physics constants of one step live in `scratch`,
+// a file-scope device array each GPU holds from the library's load until the
+// process ends; the host addresses it through the pointer hipGetSymbolAddress
+// returns on that device (the array's name is not a device address in host
+// code). Calls on one device share the array: calls that overlap on a device
+// race. Each call re-initialises what it reads before writing it (control
+// block, constants; the stage and iterate buffers are written before read).
+namespace {
+constexpr size_t NOALLOC_SCRATCH_BYTES = AMX_NOALLOC_SCRATCH_BYTES;
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch[NOALLOC_MAX_DEVICES];
+
+// Device address of this device's `scratch`, looked up once per device.
+amx_status scratch_base(unsigned char** out) {
+  int dev = -1;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail(AMX_ERR_INVALID, "device index past the scratch table%s");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return fail(AMX_ERR_HIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+    g_scratch[dev] = (unsigned char*)p;
+  }
+  *out = g_scratch[dev];
+  return AMX_OK;
+}
+
+struct ScratchLayout {
+  size_t ctl, phys, ws, amg, ppf, total;
+};
+
+ScratchLayout scratch_layout(const amx_fv_config* cfg) {
+  auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  const size_t n = (size_t)cells(cfg);
+  ScratchLayout L;
+  size_t o = 0;
+  L.ctl = o;
+  o = up(o + sizeof(Ctl));
+  L.phys = o;
+  o = up(o + sizeof(Phys));
+  L.ws = o;
+  o = up(o + sizeof(float) * n * (size_t)ws_fields(cfg, 0));
+  L.amg = o;
+  if (cfg->riemann == AMX_HYBRID_HLLC || cfg->riemann == AMX_AM_HLLC) o = up(o + n);
+  L.ppf = o;
+  if (cfg->limiter == AMX_VAN_ALBADA_PP) o = up(o + sizeof(float) * n * (size_t)(2 * cfg->dim + 1));
+  L.total = o;
+  return L;
+}
+
+}  // namespace
+
+// Control block, constants and step size of the allocation-free step: slot 0
+// live and running, B0 in the state's rows, fixed point and CFL slots cleared.
+__global__ void k_noalloc_begin(Ctl* c, Phys* ph, Phys v, const float* dtp) {
+  *ph = v;
+  c->t = 0.f;
+  c->steps = 0;
+  c->live = 1;
+  c->t_end = 0.f;
+  c->held = 0;
+  c->dt = *dtp;
+  c->run = 1;
+  c->bset = -1;
+  for (int i = 0; i < 4; ++i) c->cfl[i] = 0u;
+  zero_cflp(c);
+  fixed_point_reset(c);
+}
+
+extern "C" amx_status amx_fv_evolve_state_noalloc(const amx_fv_config* cfg, float* state, const float* d_dt,
+                                                  int* iterations, void* stream) {
+  amx_status s = validate(cfg);
+  if (s != AMX_OK) return s;
+  if (!state || !d_dt) return fail(AMX_ERR_INVALID, "null argument%s");
+  const ScratchLayout L = scratch_layout(cfg);
+  if (L.total > NOALLOC_SCRATCH_BYTES) {
+    char msg[160];
+    snprintf(msg, sizeof(msg), "this grid needs %zu bytes of scratch, the allocation-free step has %zu",
+             L.total, NOALLOC_SCRATCH_BYTES);
+    return fail(AMX_ERR_INVALID, "%s", msg);
+  }
+  unsigned char* base = nullptr;
+  s = scratch_base(&base);
+  if (s != AMX_OK) return s;
+  // host-side context on the stack: geometry only, its containers stay empty
+  amx_fv_ctx c{};
+  init_geometry(&c, cfg, 1, 0);
+  c.mem = (float*)(base + L.ws);
+  c.ctl = (Ctl*)(base + L.ctl);
+  c.dph = (Phys*)(base + L.phys);
+  c.amg = (cfg->riemann == AMX_HYBRID_HLLC || cfg->riemann == AMX_AM_HLLC) ? base + L.amg : nullptr;
+  c.ppf = cfg->limiter == AMX_VAN_ALBADA_PP ? (float*)(base + L.ppf) : nullptr;
+  Ctl hctl;
+  c.hctl = &hctl;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_noalloc_begin, dim3(1), dim3(1), 0, st, c.ctl, c.dph,
+                     make_phys(cfg->gamma, cfg->grid_spacing), d_dt);
+  HIPCHK(hipGetLastError());
+  const Bufs b = bufs(&c, state);
+  int klast = 0, K = 0;
+  s = step_phase1(&c, b, 1, st, 0, &klast);
+  if (s != AMX_OK) return s;
+  if (cfg->mhd) {
+    // the fixed point's residuals, read into host stack memory
+    HIPCHK(hipMemcpyAsync(c.hctl, c.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (!fixed_point_settled(&c, 1, &K)) {
+      s = fixed_point_more(&c, b, 1, st, &klast);
+      if (s != AMX_OK) return s;
+      fixed_point_settled(&c, 1, &K);
+    }
+    s = step_phase3(&c, b, 1, st, 0, 0);
+    if (s != AMX_OK) return s;
+  }
   if (iterations) *iterations = K;
   return AMX_OK;
 }

@@ -133,6 +133,28 @@ amx_status amx_fv_cfl_dt(amx_fv_ctx* ctx, const float* state, float* d_dt, void*
 amx_status amx_fv_evolve_state(amx_fv_ctx* ctx, float* state, const float* d_dt, int* iterations,
                                void* stream);
 
+/* Scratch of amx_fv_evolve_state_noalloc: a device array of this many bytes
+ * that every GPU the library runs on holds from first use to process exit. */
+#define AMX_NOALLOC_SCRATCH_BYTES ((size_t)128 << 20)
+
+/* amx_fv_evolve_state without a context and without any device allocation:
+ * the same step (same kernels, bit-identical result) with its workspace,
+ * control block and constants in the library's static scratch array of
+ * AMX_NOALLOC_SCRATCH_BYTES per device instead of a context's allocations.
+ * The caller passes the configuration, the state and the device step size.
+ * A grid whose workspace (about amx_fv_workspace_bytes(cfg), plus < 1 KiB
+ * of alignment) exceeds the scratch is rejected with AMX_ERR_INVALID: 128 MiB
+ * holds 3D MHD grids up to about 1.1 M cells (96^3), 2D MHD up to 1.15 M
+ * (1024^2), 3D hydro up to 6.7 M. All calls on one device share its scratch
+ * and each call re-initialises what it reads; calls that overlap on one
+ * device (other streams or threads) race and must not be issued. Like
+ * amx_fv_evolve_state it blocks once per MHD step for the fixed point's
+ * residuals (read into host stack memory). No counterpart in the reference:
+ * it is for callers that cannot allocate device memory -- code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc. */
+amx_status amx_fv_evolve_state_noalloc(const amx_fv_config* cfg, float* state, const float* d_dt,
+                                       int* iterations, void* stream);
+
 /* Device-resident step loop to t_end (or num_timesteps fixed steps), in place.
  * Replaces the loop of _time_integration (time_integration.py:629-801) with
  * snapshots off. */
