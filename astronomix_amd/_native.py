@@ -564,3 +564,10 @@ def fv_context(cfg: AmxFvConfig) -> FvContext:
 def fd_context(cfg: AmxFdConfig) -> FdContext:
     """Cached FD context per (device, configuration)."""
     return _cached("fd", FdContext, cfg)
+
+
+def clear_context_cache():
+    """Frees the cached contexts (their device workspaces)."""
+    for ctx in list(_ctx_cache.values()):
+        ctx.close()
+    _ctx_cache.clear()

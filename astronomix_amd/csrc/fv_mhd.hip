@@ -224,22 +224,21 @@ __device__ __forceinline__ void flux_hll(const float* wl, const float* wr, float
   const float Er = pr * ph.inv_gm1 + (0.5f * rr) * vsq<DIM>(wr);
   const float sp = fmaxf(fmaxf(ul + cl, ur + cr), 0.f);
   const float sm = fminf(fminf(ul - cl, ur - cr), 0.f);
-  // (S+ F_L - S- F_R + S- S+ (U_R - U_L)) / (S+ - S-) with the three weights
-  // folded once: al F_L + ar F_R + as (U_R - U_L)
+  // (S+ F_L - S- F_R + S- S+ (U_R - U_L)) / (S+ - S-) with the Euler fluxes
+  // F = u U + p (e_AX + u e_E): the conserved states enter with the weights
+  // kl = S+ (u_L - S-) / (S+ - S-) and kr = S- (S+ - u_R) / (S+ - S-) (both
+  // differences are >= the sound speeds, so no cancellation), the pressures
+  // with al = S+ / (S+ - S-) and ar = -S- / (S+ - S-)
   const float inv = frcp(sp - sm);
-  const float al = sp * inv, ar = -(sm * inv), as = (sm * sp) * inv;
-  f[0] = (al * (ul * rl) + ar * (ur * rr)) + as * (rr - rl);
+  const float kl = (sp * (ul - sm)) * inv, kr = (sm * (sp - ur)) * inv;
+  const float apl = (sp * inv) * pl, apr = -(sm * inv) * pr;
+  f[0] = kl * rl + kr * rr;
 #pragma unroll
   for (int i = 1; i <= DIM; ++i) {
-    const float ml = rl * wl[i], mr = rr * wr[i];
-    float fl = ul * ml, fr = ur * mr;
-    if (i == AX) {
-      fl += pl;
-      fr += pr;
-    }
-    f[i] = (al * fl + ar * fr) + as * (mr - ml);
+    f[i] = kl * (rl * wl[i]) + kr * (rr * wr[i]);
+    if (i == AX) f[i] += apl + apr;
   }
-  f[P] = (al * (ul * (pl + El)) + ar * (ur * (pr + Er))) + as * (Er - El);
+  f[P] = (kl * El + kr * Er) + (apl * ul + apr * ur);
 }
 
 // HLLC interface flux, central form (_finite_volume/_riemann_solver/hll.py:113-261);
@@ -839,8 +838,9 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
 // every block of the following stage kernel. k_lf_clear zeroes them first.
 constexpr int LF_BLOCKS = 512;
 
-__global__ void k_lf_clear(Ctl* ctl, int slot0) {
+__global__ void k_lf_clear(Ctl* ctl, int slot0, int gated) {
   Ctl* const c = ctl + slot0 + blockIdx.x;
+  if (gated && c->held) return;  // the stage it prepares is held too
 #pragma unroll
   for (int k = 0; k < 3; ++k) c->lfp[k][threadIdx.x] = 0u;
 }
@@ -851,7 +851,9 @@ __global__ __launch_bounds__(256) void k_lf_alpha(const StageArgs a) {
   __shared__ unsigned sred[4];
   const int slot = a.slot0 + blockIdx.y;
   Ctl* const C = a.ctl + slot;
-  if (!C->run) return;  // block-uniform
+  // block-uniform; a held (speculatively queued) stage's pre-pass waits for
+  // the launch that follows the further fixed-point iterates
+  if (!C->run || (a.gated && C->held)) return;
   const long so = slot * a.sstride;
   const int par = iterate_set(C->bset, C->iters);
   const float* src[8];
@@ -891,7 +893,9 @@ __global__ __launch_bounds__(256) void k_am_switch(const StageArgs a) {
   constexpr int P = DIM + 1;
   const int slot = a.slot0 + blockIdx.y;
   Ctl* const C = a.ctl + slot;
-  if (!C->run) return;  // block-uniform
+  // block-uniform; a held (speculatively queued) stage's pre-pass waits for
+  // the launch that follows the further fixed-point iterates
+  if (!C->run || (a.gated && C->held)) return;
   const long so = slot * a.sstride;
   const int par = iterate_set(C->bset, C->iters);
   const float* src[8];
@@ -948,7 +952,9 @@ __global__ __launch_bounds__(256) void k_pp_factors(const StageArgs a) {
   constexpr int NV = DIM + 2, P = DIM + 1;
   const int slot = a.slot0 + blockIdx.y;
   Ctl* const C = a.ctl + slot;
-  if (!C->run) return;  // block-uniform
+  // block-uniform; a held (speculatively queued) stage's pre-pass waits for
+  // the launch that follows the further fixed-point iterates
+  if (!C->run || (a.gated && C->held)) return;
   const long so = slot * a.sstride;
   const int par = iterate_set(C->bset, C->iters);
   const float* src[8];
@@ -2172,7 +2178,7 @@ hipError_t launch_stage_t(const amx_fv_ctx* c, const StageArgs& a, int nspan, hi
   const bool pp = c->cfg.limiter == AMX_VAN_ALBADA_PP;
   if (c->cfg.riemann == AMX_LAX_FRIEDRICHS || pp) {
     // the grid-wide alpha of every slot's stage input, reduced before the stage
-    hipLaunchKernelGGL(k_lf_clear, dim3(nspan), dim3(NSLOT), 0, st, a.ctl, a.slot0);
+    hipLaunchKernelGGL(k_lf_clear, dim3(nspan), dim3(NSLOT), 0, st, a.ctl, a.slot0, a.gated);
     hipLaunchKernelGGL((k_lf_alpha<DIM, SRC>), dim3(LF_BLOCKS, nspan), dim3(256), 0, st, a);
   }
   if (pp) hipLaunchKernelGGL((k_pp_factors<DIM, SRC>), dim3(LF_BLOCKS, nspan), dim3(256), 0, st, a);

@@ -6,8 +6,15 @@ A "step" is one full _evolve_state_fv step including its CFL dt reduction.
 The state stays resident in HBM for the whole timed region. Prints one JSON
 line (rank 0 only; this build runs on one GPU).
 
-    python bench.py [--steps 200] [--warmup 10] [--n 256] [--no-cpu-baseline]
+    python bench.py [--steps 200] [--warmup 10] [--n 256] [--no-cpu-baseline] [--no-serving]
     python bench.py --workload mhd_turbulence_3d_fd [--steps 100]   # FD WENO-5 + CT
+    python bench.py --workload serving_mix                          # the served mix alone
+
+The default line also carries the metric's second half, "served": solves/s and
+p50 / p99 latency of the mixed Orszag-Tang / blast request stream served with
+continuous batching (32 requests), beside the same stream one at a time.
+Per-launch kernel times (the roofline's) come from HIP events in a separate
+pass of steps after the timed region.
 """
 
 from __future__ import annotations
@@ -38,27 +45,32 @@ STAGE_BYTES_PER_CELL = 54.0
 STEP_FIELDS_STAGES = 54
 
 
-def cpu_baseline(n: int, steps: int):
+def cpu_baseline(n: int, steps: int, start=None, first_step: int = 0):
     """Time the CPU restatement (oracle/fv_oracle.cpp, float32, OpenMP on all
-    host cores of this process) on a bounded sample of the same workload."""
+    host cores of this process) on a bounded sample of the same workload:
+    `steps` steps from `start` (the GPU run's state after its warm-up, so the
+    sample is the first timed steps) or from the initial state."""
     import numpy as np
     from oracle import fv
     from astronomix_amd import workloads as W
     fv.build()
     cores = len(os.sched_getaffinity(0))
     os.environ.setdefault("OMP_NUM_THREADS", str(cores))
-    state = W.blast_3d(n)
+    state = W.blast_3d(n) if start is None else np.ascontiguousarray(start, dtype=np.float32)
     oc = fv.make_config(state.shape, 3, C_cfl=0.3, t_end=1e9)
     t0 = time.perf_counter()
-    _, done, _, _ = fv.time_integration(state, oc, max_steps=steps)
+    _, done, _, iters = fv.time_integration(state, oc, max_steps=steps)
     el = time.perf_counter() - t0
+    where = (f"steps {first_step + 1}-{first_step + done} of the benchmarked run (the first timed steps, "
+             f"from the GPU state after the warm-up)" if start is not None else f"{done} steps from the initial state")
     return {
         "value": round(done * n ** 3 / el / 1e6, 4),
         "unit": "Mzone-updates/s",
         "cores": cores,
         "kind": "port",
-        "sample": f"mhd_blast_3d {n}^3, {done} steps from the initial state, float32 C++/OpenMP "
-                  f"restatement (repo CPU restatement, not reference JAX); cpu={platform.processor() or platform.machine()}",
+        "sample": f"mhd_blast_3d {n}^3, {where}, fixed-point iterations per step {iters / max(done, 1):.2f}; "
+                  f"float32 C++/OpenMP restatement (repo CPU restatement, not reference JAX); "
+                  f"cpu={platform.processor() or platform.machine()}",
         "seconds": round(el, 3),
     }
 
@@ -112,7 +124,6 @@ def bench_fd(args):
         ctx.time_integration(state, _native.AmxFvRun(t_end=float(params.t_end), fixed_timestep=0, num_timesteps=0,
                                                      exact_end_time=1, max_steps=args.warmup))
     torch.cuda.synchronize()
-    ctx.set_timing(not args.no_kernel_timing)
     run = _native.AmxFvRun(t_end=float(params.t_end), fixed_timestep=0, num_timesteps=0, exact_end_time=1,
                            max_steps=args.steps)
     torch.cuda.synchronize()
@@ -120,8 +131,15 @@ def bench_fd(args):
     st = ctx.time_integration(state, run)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
-    weno_ms, weno_n = ctx.get_timing()
-    ctx.set_timing(False)
+    # per-launch WENO times from a separate event-timed pass after the timed region
+    weno_ms, weno_n = 0.0, 0
+    if not args.no_kernel_timing:
+        ctx.set_timing(True)
+        ctx.time_integration(state, _native.AmxFvRun(t_end=float(params.t_end), fixed_timestep=0, num_timesteps=0,
+                                                     exact_end_time=1, max_steps=min(args.timing_steps, 10)))
+        torch.cuda.synchronize()
+        weno_ms, weno_n = ctx.get_timing()
+        ctx.set_timing(False)
     if not bool(torch.isfinite(state).all()):
         print("non-finite state after the timed steps", file=sys.stderr)
         sys.exit(1)
@@ -188,28 +206,9 @@ def bench_serving(args):
     p_in ~ U[5, 50], t_end ~ U[0.1, 0.5]) against the same stream served one
     simulation at a time; both keep states on the GPU and hand back GPU
     tensors. Pools and contexts are created before the clock starts."""
-    import numpy as np
-    import torch
-    from astronomix_amd import _native
-    from astronomix_amd.serving import ServingEngine, initial_state, latency_summary, mixed_requests
-    from astronomix_amd.serving.engine import _pool_config, run_one_at_a_time
-    from astronomix_amd.serving.requests import GRIDS
-
-    reqs = mixed_requests(args.requests, rate=50.0, seed=0)
-    states = {r.rid: initial_state(r) for r in reqs}
-    eng = ServingEngine(hbm_budget=int(args.hbm_budget_gib * (1 << 30)), slots_per_pool=args.slots)
-    eng.reserve(sorted({r.grid_class for r in reqs}))
-    for kind, n in GRIDS:
-        _native.fv_context(_pool_config(kind, n))
-    torch.cuda.synchronize()
-    served = eng.serve(reqs, states, realtime=True)
-    torch.cuda.synchronize()
-    used = eng.used
-    eng.close()
-    alone = run_one_at_a_time(reqs, states, realtime=True)
-    diff = max(float((served[r.rid]["state"] - alone[r.rid]["state"]).abs().max()) for r in reqs)
-    s_served, s_alone = latency_summary(served), latency_summary(alone)
-    zone_updates = sum(served[r.rid]["steps"] * int(np.prod(states[r.rid].shape[1:])) for r in reqs)
+    res = serving_run(args)
+    s_served, s_alone = res["serving"], res["one_at_a_time"]
+    reqs = res["requests"]
     out = {
         "metric": "served solves/s (continuous batching) and p99 latency, mixed OT/blast stream",
         "value": s_served["solves_per_s"],
@@ -232,15 +231,55 @@ def bench_serving(args):
         },
         "serving": s_served,
         "one_at_a_time": s_alone,
+        "speedup_solves_per_s": res["speedup_solves_per_s"],
+        "p99_ratio_alone_over_served": res["p99_ratio_alone_over_served"],
+        "max_abs_diff_served_vs_alone": res["max_abs_diff_served_vs_alone"],
+        "zone_updates_per_s_served": res["zone_updates_per_s_served"],
+        "pool_bytes": res["pool_bytes"],
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def serving_run(args):
+    """The serving stream (args.requests requests) served with continuous
+    batching and one simulation at a time; latency summaries of both."""
+    import numpy as np
+    import torch
+    from astronomix_amd import _native
+    from astronomix_amd.serving import ServingEngine, initial_state, latency_summary, mixed_requests
+    from astronomix_amd.serving.engine import _pool_config, run_one_at_a_time
+    from astronomix_amd.serving.requests import GRIDS
+
+    reqs = mixed_requests(args.requests, rate=50.0, seed=0)
+    states = {r.rid: initial_state(r) for r in reqs}
+    eng = ServingEngine(hbm_budget=int(args.hbm_budget_gib * (1 << 30)), slots_per_pool=args.slots)
+    eng.reserve(sorted({r.grid_class for r in reqs}))
+    for kind, n in GRIDS:
+        _native.fv_context(_pool_config(kind, n))
+    torch.cuda.synchronize()
+    served = eng.serve(reqs, states, realtime=True)
+    torch.cuda.synchronize()
+    used = eng.used
+    eng.close()
+    alone = run_one_at_a_time(reqs, states, realtime=True)
+    diff = max(float((served[r.rid]["state"] - alone[r.rid]["state"]).abs().max()) for r in reqs)
+    s_served, s_alone = latency_summary(served), latency_summary(alone)
+    zone_updates = sum(served[r.rid]["steps"] * int(np.prod(states[r.rid].shape[1:])) for r in reqs)
+    del served, alone
+    _native.clear_context_cache()
+    torch.cuda.empty_cache()
+    return {
+        "requests": reqs,
+        "serving": s_served,
+        "one_at_a_time": s_alone,
         "speedup_solves_per_s": round(s_served["solves_per_s"] / s_alone["solves_per_s"], 3),
         "p99_ratio_alone_over_served": round(s_alone["p99_latency_s"] / s_served["p99_latency_s"], 3),
         "max_abs_diff_served_vs_alone": diff,
         "zone_updates_per_s_served": round(zone_updates / s_served["makespan_s"] / 1e6, 2),
         "pool_bytes": used,
-        "roofline": None,
-        "cpu_baseline": None,
     }
-    print(json.dumps(out), flush=True)
 
 
 def bench_adjoint(args):
@@ -353,8 +392,10 @@ def main():
     ap.add_argument("--hbm-budget-gib", type=float, default=32.0, help="serving_mix: pool memory budget")
     ap.add_argument("--no-kernel-timing", action="store_true",
                     help="time the steps without per-launch HIP events (no roofline figures)")
-    ap.add_argument("--timing-every", type=int, default=4,
-                    help="mhd_blast_3d: HIP events around the launches of every n-th timed step")
+    ap.add_argument("--timing-steps", type=int, default=20,
+                    help="mhd_blast_3d: steps of the event-timed pass after the timed region")
+    ap.add_argument("--no-serving", action="store_true",
+                    help="mhd_blast_3d: leave out the served-mix figures of the line")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -382,8 +423,7 @@ def main():
     if args.warmup > 0:
         ctx.time_integration(state, run_w)
     torch.cuda.synchronize()
-    every = 0 if args.no_kernel_timing else max(1, args.timing_every)
-    ctx.set_timing(every)
+    start_np = state.cpu().numpy() if not args.no_cpu_baseline else None
     run = _native.AmxFvRun(t_end=float(params.t_end), fixed_timestep=0, num_timesteps=0, exact_end_time=1,
                            max_steps=args.steps)
     torch.cuda.synchronize()
@@ -391,11 +431,23 @@ def main():
     st = ctx.time_integration(state, run)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
-    tm = ctx.get_timing()
-    ctx.set_timing(0)
     if not bool(torch.isfinite(state).all()):
         print("non-finite state after the timed steps", file=sys.stderr)
         sys.exit(1)
+    # per-launch kernel times: HIP events around every launch of a separate
+    # pass of steps that continues the run after the timed region (the timed
+    # region carries no events)
+    every = 0 if args.no_kernel_timing else 1
+    tsteps = 0
+    if every:
+        ctx.set_timing(every)
+        run_t = _native.AmxFvRun(t_end=float(params.t_end), fixed_timestep=0, num_timesteps=0, exact_end_time=1,
+                                 max_steps=args.timing_steps)
+        tst = ctx.time_integration(state, run_t)
+        torch.cuda.synchronize()
+        tsteps = tst.steps
+    tm = ctx.get_timing()
+    ctx.set_timing(0)
 
     cells = n ** 3
     steps = st.steps
@@ -403,8 +455,7 @@ def main():
     kbar = st.fixed_point_iterations / max(steps, 1)
     # executed pair launches per step (from the event records; without them,
     # the mean of floor(K / 2) + 1 is approximated by K / 2 + 1)
-    sampled = -(-steps // every) if every else 0  # steps whose launches carry events
-    pairs = tm.mag_launches / max(sampled, 1) if tm.mag_launches else kbar / 2.0 + 1.0
+    pairs = tm.mag_launches / max(tsteps, 1) if tm.mag_launches else kbar / 2.0 + 1.0
     mag_fields = MAG_FIELDS_FIRST + MAG_FIELDS_LATER * max(pairs - 1.0, 0.0)
     step_bytes = (STEP_FIELDS_STAGES + mag_fields) * 4.0
     MAG_BYTES_PER_CELL = mag_fields * 4.0 / max(pairs, 1.0)
@@ -451,8 +502,8 @@ def main():
             "traffic": traffic,
             "bytes_per_cell_per_launch": per_cell,
             "avg_launch_ms": round(avg_ms, 5),
-            "launch_timing": (f"HIP events around each launch of one timed step in {every} "
-                              f"({sampled} of {steps} steps)" if every else None),
+            "launch_timing": (f"HIP events around every launch of {tsteps} steps run right after the timed "
+                              f"region (the timed region itself carries no events)" if every else None),
         },
         "step_model": {
             "fixed_point_iterations_per_step": round(kbar, 3),
@@ -478,8 +529,26 @@ def main():
     }
     if args.copy_gib > 0:
         out["stream_copy"] = stream_copy_gbs(args.copy_gib)
+    if args.requests > 0 and not args.no_serving:
+        # the metric's second half: the served mix's solves/s and latency
+        del state
+        _native.clear_context_cache()
+        torch.cuda.empty_cache()
+        res = serving_run(args)
+        out["served"] = {
+            "workload": f"serving_mix: {args.requests} requests, Poisson 50/s seed 0; Orszag-Tang 64^2-512^2 "
+                        f"(HLL, CFL 0.4) and MHD blast 64^3-128^3 (HLL, CFL 0.3), gamma~U[1.4,5/3], "
+                        f"p_in~U[5,50], t_end~U[0.1,0.5]; {args.slots} slots per grid class",
+            "solves_per_s": res["serving"]["solves_per_s"],
+            "p50_latency_s": res["serving"]["p50_latency_s"],
+            "p99_latency_s": res["serving"]["p99_latency_s"],
+            "one_at_a_time_solves_per_s": res["one_at_a_time"]["solves_per_s"],
+            "one_at_a_time_p99_latency_s": res["one_at_a_time"]["p99_latency_s"],
+            "speedup_solves_per_s": res["speedup_solves_per_s"],
+            "max_abs_diff_served_vs_alone": res["max_abs_diff_served_vs_alone"],
+        }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(n, args.cpu_steps or 8)
+        out["cpu_baseline"] = cpu_baseline(n, args.cpu_steps or 8, start=start_np, first_step=args.warmup)
     print(json.dumps(out), flush=True)
 
 

@@ -170,3 +170,42 @@ def test_graph_replays_equal_direct_launches(gpu):
         np.testing.assert_array_equal(outs[0][r.rid], outs[1][r.rid])
         ref, _ = solo(r, state)
         np.testing.assert_array_equal(outs[0][r.rid], ref)
+
+
+def test_engine_serves_c4_grid_classes_at_the_oracle_bar(gpu, oracle):
+    """A mixed stream drawn from the serving workload's own grid classes
+    (Orszag-Tang 64^2 and 128^2, blast 64^3 with its p_in and gamma draws),
+    served with continuous batching: every served result is checked against
+    the CPU restatement at the parity bar (relative L1 <= 1e-5 per variable
+    against the float32 oracle, or within 3x of the float32 oracle's own
+    distance to float64 where that distance exceeds the bar), with the
+    oracle's step and fixed-point counts. End times are short so that the
+    oracle finishes in seconds."""
+    print("[gpu] engine C4 classes vs oracle", flush=True)
+    grids = (("orszag_tang_2d", 64), ("orszag_tang_2d", 128), ("mhd_blast_3d", 64))
+    reqs = mixed_requests(6, rate=1000.0, seed=11, grids=grids, t_end_range=(0.004, 0.012))
+    assert {r.grid_class for r in reqs} == set(grids)
+    states = {r.rid: initial_state(r) for r in reqs}
+    eng = ServingEngine(hbm_budget=4 << 30, slots_per_pool=4)
+    res = eng.serve(reqs, states, realtime=False)
+    eng.close()
+    from astronomix_amd.serving.requests import C_CFL
+
+    def rel(a, b):
+        return np.array([np.abs(a[v].astype(np.float64) - b[v]).sum() / max(np.abs(b[v]).sum(), 1e-30)
+                         for v in range(a.shape[0])])
+
+    for r in reqs:
+        dim = 2 if r.kind == "orszag_tang_2d" else 3
+        s = states[r.rid]
+        oc = oracle.make_config(s.shape, dim, mhd=True, gamma=r.gamma, C_cfl=C_CFL[r.kind], t_end=r.t_end)
+        ref, steps, t, iters = oracle.time_integration(s.astype(np.float32), oc)
+        out = res[r.rid]["state"].cpu().numpy()
+        assert res[r.rid]["steps"] == steps, (r, res[r.rid]["steps"], steps)
+        err = rel(out, ref)
+        assert np.all(np.isfinite(out))
+        if err.max() > 1e-5:
+            r64, _, _, _ = oracle.time_integration(s.astype(np.float64), oc)
+            floor, err64 = rel(ref, r64), rel(out, r64)
+            for v in np.nonzero(err > 1e-5)[0]:
+                assert floor[v] > 1e-5 / 3 and err64[v] <= 3 * floor[v], (r, v, err[v], floor[v], err64[v])
