@@ -172,6 +172,7 @@ SIGNATURES = {
     "amx_fv_pool_step_launch": (ctypes.c_int, [_P, _P]),
     "amx_fv_pool_step_finish": (ctypes.c_int, [_P, _P]),
     "amx_fv_pool_set_graphs": (ctypes.c_int, [_P, ctypes.c_int]),
+    "amx_fv_pool_set_batch": (ctypes.c_int, [_P, ctypes.c_int]),
     "amx_fv_pool_status": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(AmxFvSlotStatus)]),
     "amx_fd_create": (ctypes.c_int, [ctypes.POINTER(AmxFdConfig), ctypes.POINTER(_P)]),
     "amx_fd_destroy": (ctypes.c_int, [_P]),
@@ -377,6 +378,9 @@ class FvPool:
 
     def set_graphs(self, enable: bool):
         check(self.lib.amx_fv_pool_set_graphs(self.handle, int(bool(enable))), "amx_fv_pool_set_graphs")
+
+    def set_batch(self, steps: int):
+        check(self.lib.amx_fv_pool_set_batch(self.handle, int(steps)), "amx_fv_pool_set_batch")
 
     def step_launch(self):
         check(self.lib.amx_fv_pool_step_launch(self.handle, self._s), "amx_fv_pool_step_launch")

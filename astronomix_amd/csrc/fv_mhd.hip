@@ -38,6 +38,7 @@
 #include <functional>
 #include <map>
 #include <mutex>
+#include <tuple>
 #include <new>
 #include <utility>
 #include <vector>
@@ -72,6 +73,14 @@ struct Ctl {
   int live;    // the slot holds a simulation
   int run;     // the slot takes part in the current step
   float t_end; // end time of the slot's simulation
+  // queued steps (step loops that read back once per batch of steps): a
+  // step whose queued fixed-point launches did not settle sets stall, and
+  // the steps queued behind it do nothing until the host has finished it;
+  // the while-body iterations of the completed steps are summed on the device
+  int stall;
+  int iters_sum;
+  int iters_max;
+  int iters_bmax;  // largest count of the current batch
   unsigned cfl[4];              // reduced CFL maxima per axis
   unsigned cflp[3][NSLOT];      // CFL partial maxima
   unsigned resp[NRES][NSLOT];   // fixed-point residual partials, by launch index % NRES
@@ -1517,10 +1526,12 @@ __global__ void k_mag_final(Ctl* ctl, float tol, int max_iter, int spec) {
   const int f = pair_final(L, r0, r1, tol, max_iter);
   if (f < 0 && spec) {
     c->held = 1;
+    c->stall = 1;
     return;
   }
   c->iters = f >= 0 ? f : L + 1;
   c->held = 0;
+  c->stall = 0;
 }
 
 // Unsplit CFL maxima max(|u_a| + c) per axis (_timestep_estimator.py:118-126)
@@ -1574,6 +1585,7 @@ struct StepParams {
   int dim;
   int clamp_tp = 0;  // clamp dt to tp - t (snapshot time points, time_integration.py:671-674)
   float tp = 0.f;
+  int batch_first = 0;  // first step of a queued batch: restarts the batch's iteration maximum
 };
 
 // dt = C / sum_a(max_a / dx) (_timestep_estimator.py:128), or t_end/num_timesteps
@@ -1606,11 +1618,12 @@ __device__ __forceinline__ void zero_cflp(Ctl* c) {
 // slot runs this step (live and not yet at t_end / num_timesteps).
 __global__ void k_step_begin(Ctl* ctl, StepParams p) {
   Ctl* const c = ctl + blockIdx.x;
+  if (p.batch_first) c->iters_bmax = 0;
   if (!c->live) {
     c->run = 0;
     return;
   }
-  c->run = p.fixed ? (c->steps < p.num_timesteps) : (c->t < c->t_end);
+  c->run = !c->stall && (p.fixed ? (c->steps < p.num_timesteps) : (c->t < c->t_end));
   if (!c->run) return;
   float dt;
   if (p.fixed) {
@@ -1636,6 +1649,8 @@ __global__ void k_step_begin(Ctl* ctl, StepParams p) {
 __global__ void k_set_dt(Ctl* c, const float* dtp) {
   c->dt = *dtp;
   c->run = 1;
+  c->held = 0;
+  c->stall = 0;
   c->bset = -1;  // one step of a caller's state: B0 is its B rows
   zero_cflp(c);
   fixed_point_reset(c);
@@ -1645,9 +1660,12 @@ __global__ void k_set_dt(Ctl* c, const float* dtp) {
 // of the next step (the state's B rows are written once, by k_copy_bset).
 __global__ void k_step_end(Ctl* ctl, int keep_b) {
   Ctl* const c = ctl + blockIdx.x;
-  if (!c->run) return;
+  if (!c->run || c->held) return;  // held: the host finishes the step
   c->t = c->t + c->dt;
   c->steps += 1;
+  c->iters_sum += c->iters;
+  c->iters_max = c->iters > c->iters_max ? c->iters : c->iters_max;
+  c->iters_bmax = c->iters > c->iters_bmax ? c->iters : c->iters_bmax;
   if (keep_b) c->bset = iterate_set(c->bset, c->iters);
 }
 
@@ -1669,6 +1687,15 @@ __global__ void k_copy_bset(const Ctl* ctl, BsetArgs a) {
 }
 
 __global__ void k_bset_clear(Ctl* ctl) { ctl->bset = -1; }
+
+// The stalled steps of a queued batch (one slot per block) take part in
+// their further launches again, and only they: the steps queued behind a
+// stalled one cleared its run (k_step_begin) and changed nothing else; slots
+// that completed their steps stop running until the next step begins.
+__global__ void k_resume(Ctl* ctl) {
+  Ctl* const c = ctl + blockIdx.x;
+  c->run = c->stall ? 1 : 0;
+}
 
 __global__ void k_cfl_out(Ctl* c, float C, float dx, int dim, float* out) {
   cfl_reduce(c, dim);
@@ -1695,6 +1722,10 @@ __global__ void k_slot_reset(Ctl* ctl, int slot0, float t_end, int live) {
   c->t_end = t_end;
   c->bset = -1;
   c->held = 0;
+  c->stall = 0;
+  c->iters_sum = 0;
+  c->iters_max = 0;
+  c->iters_bmax = 0;
 }
 
 // Streaming copy, 4 floats per lane per iteration.
@@ -1787,11 +1818,13 @@ struct amx_fv_ctx {
   int klast;        // pool: last fixed-point launch of the step in flight
   int keep_b;       // the running loop keeps B in the iterate sets (no per-step copy)
   int use_graphs;   // pool: replay captured HIP graphs of the step halves
+  int batch;        // pool: steps queued by one step_launch (1: a step in two halves)
+  int batching;     // pool: the launched step is a queued batch
+  std::map<std::tuple<int, int, int>, hipGraphExec_t> g_batch;  // (span, klast, steps)
   std::map<std::pair<int, int>, hipGraphExec_t> g_launch;  // (span, klast)
   std::map<int, hipGraphExec_t> g_finish;                  // span
   std::vector<int> live;
   std::vector<int> fresh;  // loaded since the last status readback
-  std::vector<long> iters_total;
   // optional HIP-event timing of kernel launches (single simulation) on
   // every timing-th step; tstep counts steps since it was set
   int timing;
@@ -1951,6 +1984,7 @@ void free_ctx(amx_fv_ctx* c) {
   if (c->ev_status) (void)hipEventDestroy(c->ev_status);
   for (auto& g : c->g_launch) (void)hipGraphExecDestroy(g.second);
   for (auto& g : c->g_finish) (void)hipGraphExecDestroy(g.second);
+  for (auto& g : c->g_batch) (void)hipGraphExecDestroy(g.second);
   for (auto e : c->ev) (void)hipEventDestroy(e);
   delete c;
 }
@@ -2114,7 +2148,6 @@ amx_status make_ctx(const amx_fv_config* cfg, int cap, int own_state, amx_fv_ctx
   }
   c->live.assign(cap, own_state ? 0 : 1);
   c->fresh.assign(cap, 0);
-  c->iters_total.assign(cap, 0);
   *out = c;
   return AMX_OK;
 }
@@ -2668,6 +2701,10 @@ __global__ void k_noalloc_begin(Ctl* c, Phys* ph, Phys v, const float* dtp) {
   c->live = 1;
   c->t_end = 0.f;
   c->held = 0;
+  c->stall = 0;
+  c->iters_sum = 0;
+  c->iters_max = 0;
+  c->iters_bmax = 0;
   c->dt = *dtp;
   c->run = 1;
   c->bset = -1;
@@ -2778,6 +2815,14 @@ amx_status fv_loop(amx_fv_ctx* c, float* state, const amx_fv_run* run, snap::Dri
   // between steps: they are queued in batches and t is read once per batch
   // (k_step_begin clears ctl.run past the end, so surplus steps do nothing).
   const bool batch = !c->cfg.mhd && !snp && !c->hook;
+  // MHD steps of such loops are queued in batches too, each with its
+  // speculative fixed-point launches and its second half behind them; the
+  // device decides whether each step's launches settled the fixed point. A
+  // step that needs more launches stalls the batch (the steps behind it do
+  // nothing); the host reads the control block once per batch, finishes a
+  // stalled step with the further launches and goes on. Event timing keeps
+  // the step-by-step path, which sorts out early-exited launches per step.
+  const bool mbatch = c->cfg.mhd && c->keep_b && c->timing == 0;
   constexpr long BATCH = 16;
   for (;;) {
     if (run->fixed_timestep) {
@@ -2803,6 +2848,51 @@ amx_status fv_loop(amx_fv_ctx* c, float* state, const amx_fv_run* run, snap::Dri
       time_resolve(c);
       t = c->hctl[0].t;
       steps = c->hctl[0].steps;
+      continue;
+    }
+    if (mbatch) {
+      long n = BATCH;
+      if (run->max_steps >= 0 && run->max_steps - steps < n) n = run->max_steps - steps;
+      if (run->fixed_timestep && run->num_timesteps - steps < n) n = run->num_timesteps - steps;
+      int klast = 0;
+      for (long i = 0; i < n; ++i) {
+        StepParams spi = sp;
+        spi.batch_first = i == 0;
+        hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(1), 0, st, c->ctl, spi);
+        HIPCHK(hipGetLastError());
+        amx_status s = step_phase1(c, bl, 1, st, !run->fixed_timestep, &klast);
+        if (s != AMX_OK) return s;
+        s = step_phase3(c, bl, 1, st, !run->fixed_timestep, 1);
+        if (s != AMX_OK) return s;
+        hipLaunchKernelGGL(k_step_end, dim3(1), dim3(1), 0, st, c->ctl, c->keep_b);
+        HIPCHK(hipGetLastError());
+      }
+      HIPCHK(hipMemcpyAsync(c->hctl, c->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      if (c->hctl[0].stall) {
+        // the stalled step's launches up to klast have run (every launch
+        // queued behind it exited at once): the further pairs, then its
+        // second half and its end
+        int K = 0;
+        hipLaunchKernelGGL(k_resume, dim3(1), dim3(1), 0, st, c->ctl);
+        HIPCHK(hipGetLastError());
+        c->hctl[0].run = 1;
+        amx_status s = fixed_point_more(c, bl, 1, st, &klast);
+        if (s != AMX_OK) return s;
+        fixed_point_settled(c, 1, &K);
+        s = step_phase3(c, bl, 1, st, !run->fixed_timestep, 0);
+        if (s != AMX_OK) return s;
+        hipLaunchKernelGGL(k_step_end, dim3(1), dim3(1), 0, st, c->ctl, c->keep_b);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(c->hctl, c->ctl, offsetof(Ctl, cfl), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+      }
+      t = c->hctl[0].t;
+      steps = c->hctl[0].steps;
+      iters = c->hctl[0].iters_sum;
+      kmax = c->hctl[0].iters_max;
+      // the next batch queues launches for the largest count of this one
+      if (c->hctl[0].iters_bmax > 0) c->last_k = c->hctl[0].iters_bmax;
       continue;
     }
     if (snp) {
@@ -3072,6 +3162,7 @@ extern "C" amx_status amx_fv_pool_create(const amx_fv_config* cfg, const amx_fv_
   c->sp.dx = c->dx;
   c->sp.dim = cfg->dim;
   c->use_graphs = env_int("AMX_POOL_GRAPHS", 1) != 0;
+  c->batch = 1;
   // pool slots keep B in their iterate sets between steps; amx_fv_pool_read
   // writes it into the slot's state rows
   c->keep_b = c->cfg.mhd ? 1 : 0;
@@ -3118,7 +3209,6 @@ extern "C" amx_status amx_fv_pool_load(amx_fv_pool* p, int slot, const float* st
   if (!c->sp.fixed) HIPCHK(launch_cfl(c, c->mem, slot, 1, st));
   c->live[slot] = 1;
   c->fresh[slot] = 1;
-  c->iters_total[slot] = 0;
   return AMX_OK;
 }
 
@@ -3176,6 +3266,40 @@ extern "C" amx_status amx_fv_pool_step_launch(amx_fv_pool* p, void* stream) {
     if (upto > maxit) upto = maxit;
     klast = (upto / 2) * 2 + 1;
   }
+  if (c->batch > 1) {
+    // c->batch whole steps queued at once, each with its second half behind
+    // its speculative launches; the device decides per slot whether they
+    // settled the fixed point (a slot whose step stalls does nothing until
+    // step_finish has completed that step)
+    const int nb = c->batch;
+    const auto bkey = std::make_tuple(span, klast, nb);
+    auto bit = c->g_batch.find(bkey);
+    hipGraphExec_t bx = bit == c->g_batch.end() ? nullptr : bit->second;
+    amx_status bs = replay_or_capture(c, st, &bx, bx != nullptr, [&]() -> amx_status {
+      for (int i = 0; i < nb; ++i) {
+        StepParams spi = c->sp;
+        spi.batch_first = i == 0;
+        hipLaunchKernelGGL(k_step_begin, dim3(span), dim3(1), 0, st, c->ctl, spi);
+        HIPCHK(hipGetLastError());
+        int kl = 0;
+        amx_status r = step_phase1(c, b, span, st, !c->sp.fixed, &kl);
+        if (r != AMX_OK) return r;
+        r = step_phase3(c, b, span, st, !c->sp.fixed, 1);
+        if (r != AMX_OK) return r;
+        hipLaunchKernelGGL(k_step_end, dim3(span), dim3(1), 0, st, c->ctl, c->keep_b);
+        HIPCHK(hipGetLastError());
+      }
+      return AMX_OK;
+    });
+    if (bs != AMX_OK) return bs;
+    if (bx) c->g_batch[bkey] = bx;
+    c->klast = klast;
+    HIPCHK(hipMemcpyAsync(c->hctl, c->ctl, sizeof(Ctl) * span, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipEventRecord(c->ev_status, st));
+    c->stepping = 1;
+    c->batching = 1;
+    return AMX_OK;
+  }
   const auto key = std::make_pair(span, klast);
   auto it = c->g_launch.find(key);
   hipGraphExec_t x = it == c->g_launch.end() ? nullptr : it->second;
@@ -3205,13 +3329,40 @@ extern "C" amx_status amx_fv_pool_step_finish(amx_fv_pool* p, void* stream) {
   const Bufs b = bufs(c, nullptr);
   int klast = c->klast;
   int kmax = 0;
+  if (c->batching) {
+    c->batching = 0;
+    int stalled = 0, bmax = 0;
+    for (int i = 0; i < span; ++i) {
+      stalled |= c->hctl[i].stall;
+      bmax = c->hctl[i].iters_bmax > bmax ? c->hctl[i].iters_bmax : bmax;
+    }
+    if (stalled) {
+      // complete the stalled slots' steps (the others sit out: run = 0)
+      hipLaunchKernelGGL(k_resume, dim3(span), dim3(1), 0, st, c->ctl);
+      HIPCHK(hipGetLastError());
+      for (int i = 0; i < span; ++i) c->hctl[i].run = c->hctl[i].stall;
+      amx_status s = fixed_point_more(c, b, span, st, &klast);
+      if (s != AMX_OK) return s;
+      int km = 0;
+      fixed_point_settled(c, span, &km);
+      bmax = km > bmax ? km : bmax;
+      s = step_phase3(c, b, span, st, !c->sp.fixed, 0);
+      if (s != AMX_OK) return s;
+      hipLaunchKernelGGL(k_step_end, dim3(span), dim3(1), 0, st, c->ctl, c->keep_b);
+      HIPCHK(hipGetLastError());
+    }
+    if (c->cfg.mhd && bmax > 0) c->last_k = bmax;
+    HIPCHK(hipMemcpyAsync(c->hctl, c->ctl, sizeof(Ctl) * span, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipEventRecord(c->ev_status, st));
+    for (int i = 0; i < span; ++i) c->fresh[i] = 0;
+    return AMX_OK;
+  }
   if (c->cfg.mhd) {
     amx_status s = fixed_point_more(c, b, span, st, &klast);
     if (s != AMX_OK) return s;
     fixed_point_settled(c, span, &kmax);
     c->last_k = kmax;
-    for (int i = 0; i < span; ++i)
-      if (c->hctl[i].run) c->iters_total[i] += slot_final(c, c->hctl[i]);
+
   }
   auto it = c->g_finish.find(span);
   hipGraphExec_t x = it == c->g_finish.end() ? nullptr : it->second;
@@ -3237,6 +3388,14 @@ extern "C" amx_status amx_fv_pool_set_graphs(amx_fv_pool* p, int enable) {
   return AMX_OK;
 }
 
+extern "C" amx_status amx_fv_pool_set_batch(amx_fv_pool* p, int steps) {
+  if (!p || !p->c) return fail(AMX_ERR_INVALID, "null pool%s");
+  if (p->c->stepping) return fail(AMX_ERR_INVALID, "a step is in flight%s");
+  if (steps < 1 || steps > 256) return fail(AMX_ERR_INVALID, "steps per launch must be in 1..256%s");
+  p->c->batch = steps;
+  return AMX_OK;
+}
+
 extern "C" amx_status amx_fv_pool_status(amx_fv_pool* p, int slot, amx_fv_slot_status* out) {
   if (!p || !p->c || !out) return fail(AMX_ERR_INVALID, "null argument%s");
   amx_fv_ctx* c = p->c;
@@ -3254,7 +3413,7 @@ extern "C" amx_status amx_fv_pool_status(amx_fv_pool* p, int slot, amx_fv_slot_s
   out->steps = h.steps;
   out->t = h.t;
   out->t_end = h.t_end;
-  out->fixed_point_iterations = c->iters_total[slot];
+  out->fixed_point_iterations = h.iters_sum;
   out->finished = c->sp.fixed ? (h.steps >= c->sp.num_timesteps) : !(h.t < h.t_end);
   return AMX_OK;
 }

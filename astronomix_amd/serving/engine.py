@@ -28,10 +28,13 @@ class ServingEngine:
     when it is empty and another class is waiting for memory).
     """
 
-    def __init__(self, hbm_budget: int = 24 << 30, slots_per_pool: int = 16):
+    def __init__(self, hbm_budget: int = 24 << 30, slots_per_pool: int = 16, steps_per_launch: int = 16):
         _native.require_gpu()
         self.budget = int(hbm_budget)
         self.slots_per_pool = int(slots_per_pool)
+        # whole steps queued per pool launch (one host readback per batch);
+        # slots that finish inside a batch wait for its end
+        self.steps_per_launch = int(steps_per_launch)
         self.pools = {}      # grid class -> FvPool
         self.slot_req = {}   # grid class -> [Request | None] per slot
         self.pool_bytes = {}
@@ -48,6 +51,8 @@ class ServingEngine:
         if cap < 1:
             return None
         pool = _native.FvPool(_pool_config(*cls), cap)
+        if self.steps_per_launch > 1:
+            pool.set_batch(self.steps_per_launch)
         self.pools[cls] = pool
         self.slot_req[cls] = [None] * cap
         self.pool_bytes[cls] = cap * sb
@@ -91,8 +96,9 @@ class ServingEngine:
 
     # ---- stepping
     def step(self):
-        """One step of every pool with running slots; returns the finished
-        (request, state tensor) pairs and frees their slots."""
+        """One launch (steps_per_launch steps) of every pool with running
+        slots; returns the finished (request, state tensor) pairs and frees
+        their slots."""
         import torch
         active = [cls for cls, slots in self.slot_req.items() if any(r is not None for r in slots)]
         for cls in active:

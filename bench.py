@@ -214,7 +214,7 @@ def bench_serving(args):
         "value": s_served["solves_per_s"],
         "unit": "solves/s",
         "n_gpus": 1,
-        "steps": int(sum(served[r.rid]["steps"] for r in reqs)),
+        "steps": res["steps"],
         "warmup": 0,
         "ms_per_step": None,
         "higher_is_better": True,
@@ -226,6 +226,7 @@ def bench_serving(args):
             "workload": f"serving_mix: {len(reqs)} requests, Poisson 50/s seed 0; Orszag-Tang 64^2-512^2 (HLL, CFL 0.4) "
                         f"and MHD blast 64^3-128^3 (HLL, CFL 0.3), gamma~U[1.4,5/3], p_in~U[5,50], t_end~U[0.1,0.5]",
             "slots_per_pool": args.slots,
+            "steps_per_launch": args.steps_per_launch,
             "hbm_budget_gib": args.hbm_budget_gib,
             "parallelism": "single GPU",
         },
@@ -254,7 +255,8 @@ def serving_run(args):
 
     reqs = mixed_requests(args.requests, rate=50.0, seed=0)
     states = {r.rid: initial_state(r) for r in reqs}
-    eng = ServingEngine(hbm_budget=int(args.hbm_budget_gib * (1 << 30)), slots_per_pool=args.slots)
+    eng = ServingEngine(hbm_budget=int(args.hbm_budget_gib * (1 << 30)), slots_per_pool=args.slots,
+                        steps_per_launch=args.steps_per_launch)
     eng.reserve(sorted({r.grid_class for r in reqs}))
     for kind, n in GRIDS:
         _native.fv_context(_pool_config(kind, n))
@@ -267,11 +269,13 @@ def serving_run(args):
     diff = max(float((served[r.rid]["state"] - alone[r.rid]["state"]).abs().max()) for r in reqs)
     s_served, s_alone = latency_summary(served), latency_summary(alone)
     zone_updates = sum(served[r.rid]["steps"] * int(np.prod(states[r.rid].shape[1:])) for r in reqs)
+    steps = int(sum(served[r.rid]["steps"] for r in reqs))
     del served, alone
     _native.clear_context_cache()
     torch.cuda.empty_cache()
     return {
         "requests": reqs,
+        "steps": steps,
         "serving": s_served,
         "one_at_a_time": s_alone,
         "speedup_solves_per_s": round(s_served["solves_per_s"] / s_alone["solves_per_s"], 3),
@@ -390,6 +394,8 @@ def main():
     ap.add_argument("--requests", type=int, default=32, help="serving_mix: requests in the stream")
     ap.add_argument("--slots", type=int, default=16, help="serving_mix: slots per grid-class pool")
     ap.add_argument("--hbm-budget-gib", type=float, default=32.0, help="serving_mix: pool memory budget")
+    ap.add_argument("--steps-per-launch", type=int, default=16,
+                    help="serving_mix: whole steps a pool queues per launch (one readback per batch)")
     ap.add_argument("--no-kernel-timing", action="store_true",
                     help="time the steps without per-launch HIP events (no roofline figures)")
     ap.add_argument("--timing-steps", type=int, default=20,

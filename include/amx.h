@@ -239,6 +239,15 @@ amx_status amx_fv_pool_step_finish(amx_fv_pool* pool, void* stream);
  * batch) by default (env AMX_POOL_GRAPHS=0 turns this off at creation);
  * enable = 0 launches the same kernels directly. */
 amx_status amx_fv_pool_set_graphs(amx_fv_pool* pool, int enable);
+/* Steps per _launch / _finish (default 1). With steps > 1, _launch queues
+ * that many whole steps of every running slot, each step's second half
+ * queued behind its fixed-point launches, and the device decides per slot
+ * whether those launches settled the fixed point; a slot whose step did not
+ * settle does nothing more in the batch, and _finish completes that step with
+ * the further launches. One host readback per batch instead of two per step;
+ * the results are those of single steps, bit for bit. Slots that finish
+ * inside a batch wait for its end. */
+amx_status amx_fv_pool_set_batch(amx_fv_pool* pool, int steps);
 /* Status of a slot after the last finished step (waits for its readback). */
 amx_status amx_fv_pool_status(amx_fv_pool* pool, int slot, amx_fv_slot_status* out);
 

@@ -209,3 +209,60 @@ def test_engine_serves_c4_grid_classes_at_the_oracle_bar(gpu, oracle):
             floor, err64 = rel(ref, r64), rel(out, r64)
             for v in np.nonzero(err > 1e-5)[0]:
                 assert floor[v] > 1e-5 / 3 and err64[v] <= 3 * floor[v], (r, v, err[v], floor[v], err64[v])
+
+
+@pytest.mark.parametrize("batch", [5, 16])
+def test_batched_pool_steps_equal_single_path(gpu, batch):
+    """Pools that queue several whole steps per launch (the device decides
+    each step's fixed point, one readback per batch): every slot equals the
+    single-simulation path bit for bit, with slots that finish inside a batch
+    and a batch size that does not divide the step counts."""
+    import torch
+    print(f"[gpu] batched pool 2D OT 64^2, {batch} steps per launch", flush=True)
+    reqs = [Request(0, "orszag_tang_2d", 64, 1.4, 0.05, 0, 0), Request(1, "orszag_tang_2d", 64, 5 / 3, 0.08, 0, 0),
+            Request(2, "orszag_tang_2d", 64, 1.55, 0.03, 0, 0)]
+    pool = _native.FvPool(_pool_config("orszag_tang_2d", 64), 4)
+    pool.set_batch(batch)
+    state = W.orszag_tang_2d(64)
+    for i, r in enumerate(reqs):
+        pool.load(i, state, r.gamma, r.t_end)
+    st = step_until_done(pool, [0, 1, 2])
+    for i, r in enumerate(reqs):
+        out = pool.read(i, torch.empty(state.shape, dtype=torch.float32, device="cuda")).cpu().numpy()
+        ref, rst = solo(r, state)
+        assert st[i].steps == rst.steps and st[i].fixed_point_iterations == rst.fixed_point_iterations
+        np.testing.assert_array_equal(out, ref)
+    pool.close()
+
+
+def test_batched_pool_completes_stalled_steps(gpu):
+    """At fixed-point tolerance 1e-6 the first steps need more launches than
+    the batch queues: those slots stall, the rest of the batch does nothing
+    for them, and step_finish completes their steps. Two 3D slots with
+    different gamma and t_end equal their single-simulation loops (which
+    stall and complete the same way) bit for bit."""
+    import torch
+    print("[gpu] batched pool with stalled fixed points", flush=True)
+    state = W.random_smooth((24, 24, 24), True, seed=1, amp=1.0)
+    cfg = _pool_config("mhd_blast_3d", 24)
+    cfg.fixed_point_tol = 1e-6
+    cfg.C_cfl = 0.8
+    pool = _native.FvPool(cfg, 2)
+    pool.set_batch(6)
+    runs = [(5 / 3, 0.012), (1.5, 0.02)]
+    for i, (g, te) in enumerate(runs):
+        pool.load(i, state, g, te)
+    st = step_until_done(pool, [0, 1])
+    for i, (g, te) in enumerate(runs):
+        out = pool.read(i, torch.empty(state.shape, dtype=torch.float32, device="cuda")).cpu().numpy()
+        c1 = _native.AmxFvConfig.from_buffer_copy(cfg)
+        c1.gamma = g
+        ctx = _native.FvContext(c1)
+        s = torch.from_numpy(np.ascontiguousarray(state)).cuda()
+        rst = ctx.time_integration(s, _native.AmxFvRun(t_end=te, fixed_timestep=0, num_timesteps=0, exact_end_time=1,
+                                                       max_steps=-1))
+        torch.cuda.synchronize()
+        assert st[i].steps == rst.steps and st[i].fixed_point_iterations == rst.fixed_point_iterations
+        assert rst.max_fixed_point_iterations > 5
+        np.testing.assert_array_equal(out, s.cpu().numpy())
+    pool.close()
