@@ -39,28 +39,38 @@ class FdConfig(ctypes.Structure):
     ]
 
 
+def use_library(path=None):
+    """Bind the functions below to the restatement built at `path` (None: the
+    default build). The mutation tests bind copies with planted mistakes."""
+    global _lib
+    _lib = None if path is None else _bind(ctypes.CDLL(path))
+
+
 def _load():
     global _lib
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
             _fv.build()
-        lib = ctypes.CDLL(_LIB_PATH)
-        P = ctypes.c_void_p
-        C = ctypes.POINTER(FdConfig)
-        for s in ("f32", "f64"):
-            getattr(lib, f"fd_time_integration_{s}").argtypes = [C, P, ctypes.c_long, ctypes.POINTER(ctypes.c_double)]
-            getattr(lib, f"fd_time_integration_{s}").restype = ctypes.c_long
-            getattr(lib, f"fd_evolve_state_{s}").argtypes = [C, P, ctypes.c_double]
-            getattr(lib, f"fd_cfl_dt_{s}").argtypes = [C, P]
-            getattr(lib, f"fd_cfl_dt_{s}").restype = ctypes.c_double
-            getattr(lib, f"fd_weno_flux_{s}").argtypes = [C, P, ctypes.c_int, P]
-            getattr(lib, f"fd_compute_rhs_{s}").argtypes = [C, P, P, ctypes.c_double, ctypes.c_double, P, P]
-            getattr(lib, f"fd_eigensystem_{s}").argtypes = [C, P, P, P, P, P]
-            getattr(lib, f"fd_mhd_flux_{s}").argtypes = [C, P, P]
-            getattr(lib, f"fd_face_divergence_{s}").argtypes = [C, P, P, P, P]
-            getattr(lib, f"fd_interface_fields_{s}").argtypes = [C, P, P]
-        _lib = lib
+        _lib = _bind(ctypes.CDLL(_LIB_PATH))
     return _lib
+
+
+def _bind(lib):
+    P = ctypes.c_void_p
+    C = ctypes.POINTER(FdConfig)
+    for s in ("f32", "f64"):
+        getattr(lib, f"fd_time_integration_{s}").argtypes = [C, P, ctypes.c_long, ctypes.POINTER(ctypes.c_double)]
+        getattr(lib, f"fd_time_integration_{s}").restype = ctypes.c_long
+        getattr(lib, f"fd_evolve_state_{s}").argtypes = [C, P, ctypes.c_double]
+        getattr(lib, f"fd_cfl_dt_{s}").argtypes = [C, P]
+        getattr(lib, f"fd_cfl_dt_{s}").restype = ctypes.c_double
+        getattr(lib, f"fd_weno_flux_{s}").argtypes = [C, P, ctypes.c_int, P]
+        getattr(lib, f"fd_compute_rhs_{s}").argtypes = [C, P, P, ctypes.c_double, ctypes.c_double, P, P]
+        getattr(lib, f"fd_eigensystem_{s}").argtypes = [C, P, P, P, P, P]
+        getattr(lib, f"fd_mhd_flux_{s}").argtypes = [C, P, P]
+        getattr(lib, f"fd_face_divergence_{s}").argtypes = [C, P, P, P, P]
+        getattr(lib, f"fd_interface_fields_{s}").argtypes = [C, P, P]
+    return lib
 
 
 def _sfx(a) -> str:

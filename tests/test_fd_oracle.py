@@ -262,3 +262,53 @@ def test_float32_tracks_float64(fd):
     o32 = fd.time_integration(s.astype(np.float32), c)[0]
     rel = np.abs(o32 - o64).sum(axis=(1, 2, 3)) / np.maximum(np.abs(o64).sum(axis=(1, 2, 3)), 1e-30)
     assert rel.max() < 2e-5, rel
+
+
+def ct_scaled_error(fd):
+    """rhs_bx / dt + V b' of B_x = b(z) advected by v = (0, 0, V) (B_z = B0),
+    as a multiple of the h^2 term -V h^2 b_zzz / 24, at n = 32, 64, 128."""
+    V, B0, A = 0.7, 0.5, 0.2
+    ratios = []
+    for n in (32, 64, 128):
+        h = 1.0 / n
+        z = (np.arange(n) + 0.5) * h
+        k = 2 * np.pi
+        b = A * np.sin(k * z)
+        shape = (6, 6, n)
+        q = np.zeros((8,) + shape)
+        q[0] = 1.0
+        q[3] = V
+        q[5] = b
+        q[7] = B0
+        q[4] = 1.0 / (GAMMA - 1) + 0.5 * V * V + 0.5 * (b ** 2 + B0 ** 2)
+        faces = np.zeros((3,) + shape)
+        faces[0] = b
+        faces[2] = B0
+        c = fd.make_config(shape)
+        _, rb = fd.compute_rhs(q, faces, c, 1.0, 1.0)
+        err = rb[0] + V * A * k * np.cos(k * z)
+        lead = -V * h * h * (-A * k ** 3 * np.cos(k * z)) / 24.0
+        ratios.append(float((err * lead).sum() / (lead * lead).sum()))
+        # nothing but B_x changes: the y and z faces stay
+        assert np.abs(rb[1]).max() < 1e-12 and np.abs(rb[2]).max() < 1e-12
+    return ratios
+
+
+def test_ct_point_to_average_weight_is_one_24th(fd):
+    """The CT edge EMFs are converted from point values to edge averages by
+    q + (delta^2_A q + delta^2_B q) / 24 (_interpolate.py:38-62,
+    _constrained_transport.py:140-142) before the 6th-order difference
+    (_differencing.py:8-25). The finite-difference WENO face fluxes are
+    primitive-function values, f - h^2 f_xx / 24 + O(h^4) for a smooth flux
+    f (their differences give f_x to 5th order), so the edge EMFs carry an
+    O(h^2) term that the conversion must cancel: only with the weight 1/24
+    is the face update of an advected B_x = b(z) (v = (0, 0, V), B_z = B0)
+    4th-order accurate; any other weight leaves an O(h^2) error whose size
+    is proportional to (24 w - 1). (Planted weights 1/12, 1/48 and 0 give a
+    scaled error of +36, -18 and -36 in the units below, constant in h:
+    second order.)"""
+    ratios = ct_scaled_error(fd)
+    # the error in units of h^2 b_zzz vanishes as h -> 0 ...
+    assert abs(ratios[-1]) < 0.01, ratios
+    # ... at 4th order: the h^2-scaled error falls with h^2 at the fine end
+    assert abs(ratios[-1]) < abs(ratios[-2]) / 2.5, ratios

@@ -4,8 +4,11 @@ fail at least one of the pins in tests/test_oracle_pins.py. This shows that
 the pins fix the parts of the restatement the reference's golden output
 cannot: the reconstruction (slope neighbours, dx/2, primitive variables),
 the Heun average, the HLLC star speed and the signs of the magnetic
-sub-step's Lorentz force and induction curl. The mistakes are planted in the
-oracle on the CPU only; nothing here touches the GPU."""
+sub-step's Lorentz force and induction curl; and, in copies of the
+finite-difference restatement (oracle/fd_oracle.cpp), the point-to-average
+weight of the CT edge EMFs, which tests/test_fd_oracle.py pins by the order
+of the face update. The mistakes are planted in the oracle on the CPU only;
+nothing here touches the GPU."""
 
 import os
 import subprocess
@@ -138,3 +141,46 @@ def test_planted_mistake_fails_a_pin(oracle, mutant_libs, name):
         assert caught, f"planted mistake {name} passed the pins {pins}"
     finally:
         oracle.use_library(None)
+
+
+FD_SRC = os.path.join(ROOT, "oracle", "fd_oracle.cpp")
+P2A = """  const T sa = ((q[g.nb(c, A, -1)] - T(2) * q[c]) + q[g.nb(c, A, 1)]) / T(24);
+  const T sb = ((q[g.nb(c, B, -1)] - T(2) * q[c]) + q[g.nb(c, B, 1)]) / T(24);"""
+FD_MUTANTS = {
+    "ct_weight_one_12th": P2A.replace("/ T(24)", "/ T(12)"),
+    "ct_weight_one_48th": P2A.replace("/ T(24)", "/ T(48)"),
+    "ct_no_point_to_average": P2A.replace("/ T(24)", "* T(0)"),
+}
+
+
+@pytest.fixture(scope="module")
+def fd_mutant_libs(tmp_path_factory):
+    src = open(FD_SRC).read()
+    assert src.count(P2A) == 1, "the planted site is no longer in fd_oracle.cpp"
+    d = tmp_path_factory.mktemp("fd_mutants")
+    procs = {}
+    for name, planted in FD_MUTANTS.items():
+        cpp = d / f"{name}.cpp"
+        cpp.write_text(src.replace(P2A, planted))
+        so = d / f"lib{name}.so"
+        procs[name] = (so, subprocess.Popen(
+            ["g++", "-O1", "-fopenmp", "-ffp-contract=off", "-fno-fast-math", "-fPIC", "-std=c++17",
+             "-shared", "-o", str(so), str(cpp)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT))
+    libs = {}
+    for name, (so, p) in procs.items():
+        out, _ = p.communicate(timeout=300)
+        assert p.returncode == 0, out.decode()
+        libs[name] = str(so)
+    return libs
+
+
+@pytest.mark.parametrize("name", sorted(FD_MUTANTS))
+def test_planted_ct_weight_fails_the_order_pin(oracle, fd_mutant_libs, name):
+    from oracle import fd
+    import test_fd_oracle as T
+    fd.use_library(fd_mutant_libs[name])
+    try:
+        with pytest.raises(AssertionError):
+            T.test_ct_point_to_average_weight_is_one_24th(fd)
+    finally:
+        fd.use_library(None)
