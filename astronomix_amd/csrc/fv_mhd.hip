@@ -1066,6 +1066,9 @@ struct MagArgs {
   long sstride;        // floats between consecutive slots' buffers
   int slot0;
   int L;               // first launch index of the pair (even)
+  int write_first;     // store the pair's first iterate (else only a regeneration would)
+  int regen;           // regeneration launch: see k_mag
+  unsigned skip_mask;  // regeneration: pairs (bit L / 2) queued without storing their first iterate
   int max_iter;
   float tol;
   int euler;           // IMPLICIT_EULER: average = iterate
@@ -1229,8 +1232,20 @@ __global__ __launch_bounds__(((TA + 4) * (TBM + 4) + 255) / 256 * 256) void k_ma
   const int slot = a.slot0 + blockIdx.y;
   Ctl* const C = a.ctl + slot;
   if (!C->run) return;
-  const int L = a.L;
-  if (L > 0) {
+  int L = a.L;
+  if (a.regen) {
+    // Regeneration: the fixed point's final launch is read from the residuals
+    // of the last executed pair; if it is that pair's first launch and the
+    // pair did not store its iterate, the launch is computed again (the same
+    // arithmetic on the same inputs, so bit-identical) and only that iterate
+    // is stored. Residuals, counters and flags are left as they are.
+    const int Ll = C->iters - 1;
+    if (Ll < 0 || Ll > 62) return;
+    const float r0 = __uint_as_float(wave_max_u(C->resp[Ll % NRES][tid & 63]));
+    const float r1 = __uint_as_float(wave_max_u(C->resp[(Ll + 1) % NRES][tid & 63]));
+    if (pair_final(Ll, r0, r1, a.tol, a.max_iter) != Ll || !((a.skip_mask >> (Ll >> 1)) & 1u)) return;
+    L = Ll;
+  } else if (L > 0) {
     if (C->done) return;
     const float r0 = __uint_as_float(wave_max_u(C->resp[(L - 2) % NRES][tid & 63]));
     const float r1 = __uint_as_float(wave_max_u(C->resp[(L - 1) % NRES][tid & 63]));
@@ -1242,11 +1257,12 @@ __global__ __launch_bounds__(((TA + 4) * (TBM + 4) + 255) / 256 * 256) void k_ma
   __shared__ float sK[2][6][NC];
   __shared__ float sK1[2][6][NC1];
   __shared__ unsigned sred[2][NW];
-  if (blockIdx.x == 0 && tid < NSLOT) {
+  if (!a.regen && blockIdx.x == 0 && tid < NSLOT) {
     if (tid == 0) C->iters = L + 1;
     C->resp[(L + 2) % NRES][tid] = 0u;
     C->resp[(L + 3) % NRES][tid] = 0u;
   }
+  const bool store1 = a.write_first || a.regen;
   const long so = slot * a.sstride;
   MagArgs m = a;  // slot-local pointers
   // B0 is in the state's rows or, in a loop that keeps B in the iterate
@@ -1344,10 +1360,12 @@ __global__ __launch_bounds__(((TA + 4) * (TBM + 4) + 255) / 256 * 256) void k_ma
       mag_curls<2, WX>(SK, o, nullptr, nullptr, idx, cE, cB);
       mag_apply(d, cE, cB, rc.B0, rc.v0, rho, dt, B1, v1);
       if (own) {
+        if (store1) {
 #pragma unroll
-        for (int i = 0; i < 3; ++i) {
-          st32(m.Bo[0][i], col, B1[i]);
-          st32(m.vo[0][i], col, v1[i]);
+          for (int i = 0; i < 3; ++i) {
+            st32(m.Bo[0][i], col, B1[i]);
+            st32(m.vo[0][i], col, v1[i]);
+          }
         }
         const float rr = mag_resid(rc.Bk, rc.vk, B1, v1);
         res0 = (rr != rr || rr > res0) ? rr : res0;
@@ -1360,7 +1378,7 @@ __global__ __launch_bounds__(((TA + 4) * (TBM + 4) + 255) / 256 * 256) void k_ma
       if (BCX && own) ghost_copies(sK1[0], e1);
     }
     __syncthreads();
-    if (own) {
+    if (own && !a.regen) {
       float cE[3], cB[3], e1[6], B2[3], v2[3];
 #pragma unroll
       for (int i = 0; i < 6; ++i) {
@@ -1430,10 +1448,12 @@ __global__ __launch_bounds__(((TA + 4) * (TBM + 4) + 255) / 256 * 256) void k_ma
         mag_apply(dc, cE, cB, rc.B0, rc.v0, rho_c, dt, B1, v1);
         if (own && x >= x0 && x < x1) {
           const unsigned cell = (unsigned)x * sx + col;
+          if (store1) {
 #pragma unroll
-          for (int i = 0; i < 3; ++i) {
-            st32(m.Bo[0][i], cell, B1[i]);
-            st32(m.vo[0][i], cell, v1[i]);
+            for (int i = 0; i < 3; ++i) {
+              st32(m.Bo[0][i], cell, B1[i]);
+              st32(m.vo[0][i], cell, v1[i]);
+            }
           }
           const float rr = mag_resid(rc.Bk, rc.vk, B1, v1);
           res0 = (rr != rr || rr > res0) ? rr : res0;
@@ -1446,7 +1466,7 @@ __global__ __launch_bounds__(((TA + 4) * (TBM + 4) + 255) / 256 * 256) void k_ma
         if (BCX && own) ghost_copies(sK1[bx], e1);
       }
       __syncthreads();
-      if (own && x - 1 >= x0 && x - 1 < x1) {
+      if (own && !a.regen && x - 1 >= x0 && x - 1 < x1) {
         float cE[3], cB[3], B2[3], v2[3];
 #pragma unroll
         for (int i = 0; i < 6; ++i) SK1[i] = sK1[bx ^ 1][i];
@@ -1505,7 +1525,7 @@ __global__ __launch_bounds__(((TA + 4) * (TBM + 4) + 255) / 256 * 256) void k_ma
   }
   const unsigned m0 = block_max_u<NW>(canon_bits(res0), sred[0]);
   const unsigned m1 = block_max_u<NW>(canon_bits(res1), sred[1]);
-  if (tid == 0) {
+  if (tid == 0 && !a.regen) {
     if (m0) atomicMax(&C->resp[L % NRES][blockIdx.x & (NSLOT - 1)], m0);
     if (m1) atomicMax(&C->resp[(L + 1) % NRES][blockIdx.x & (NSLOT - 1)], m1);
   }
@@ -1818,11 +1838,12 @@ struct amx_fv_ctx {
   int klast;        // pool: last fixed-point launch of the step in flight
   int keep_b;       // the running loop keeps B in the iterate sets (no per-step copy)
   int use_graphs;   // pool: replay captured HIP graphs of the step halves
+  unsigned skip_mask;  // pairs of the current step queued without storing their first iterate
   int batch;        // pool: steps queued by one step_launch (1: a step in two halves)
   int batching;     // pool: the launched step is a queued batch
-  std::map<std::tuple<int, int, int>, hipGraphExec_t> g_batch;  // (span, klast, steps)
-  std::map<std::pair<int, int>, hipGraphExec_t> g_launch;  // (span, klast)
-  std::map<int, hipGraphExec_t> g_finish;                  // span
+  std::map<std::tuple<int, int, int>, hipGraphExec_t> g_batch;  // (span, last count, steps)
+  std::map<std::pair<int, int>, hipGraphExec_t> g_launch;  // (span, last count)
+  std::map<std::pair<int, unsigned>, hipGraphExec_t> g_finish;  // (span, pairs without first iterate)
   std::vector<int> live;
   std::vector<int> fresh;  // loaded since the last status readback
   // optional HIP-event timing of kernel launches (single simulation) on
@@ -2277,6 +2298,9 @@ MagArgs mag_base(const amx_fv_ctx* c) {
   m.sstride = c->sstride;
   m.slot0 = 0;
   m.max_iter = c->cfg.max_fixed_point_iterations;
+  m.write_first = 1;
+  m.regen = 0;
+  m.skip_mask = 0u;
   m.tol = c->cfg.fixed_point_tol;
   m.euler = c->cfg.magnetic_integrator == AMX_IMPLICIT_EULER;
   m.nx = c->nxm;
@@ -2332,9 +2356,13 @@ hipError_t launch_cfl(const amx_fv_ctx* c, const float* state, int slot0, int ns
 // Launches the fixed-point pair (L, L+1): input = launch L-1's output (the
 // state's B and gas velocity for L = 0); the kernel picks the iterate sets
 // from ctl->bset (iterate_set).
-hipError_t launch_mag_pair(amx_fv_ctx* c, const Bufs& b, int L, int nspan, hipStream_t st) {
+hipError_t launch_mag_pair(amx_fv_ctx* c, const Bufs& b, int L, int nspan, hipStream_t st, int write_first = 1,
+                           int regen = 0) {
   const int dim = c->cfg.dim;
   MagArgs m = mag_base(c);
+  m.write_first = write_first;
+  m.regen = regen;
+  m.skip_mask = c->skip_mask;
   for (int k = 0; k < 3; ++k) m.B0[k] = b.B[k];
   m.v0[0] = b.gas[1];
   m.v0[1] = b.gas[2];
@@ -2343,10 +2371,21 @@ hipError_t launch_mag_pair(amx_fv_ctx* c, const Bufs& b, int L, int nspan, hipSt
   for (int j = 0; j < 4; ++j)
     for (int k = 0; k < 6; ++k) m.PS[j][k] = b.P[j][k];
   m.L = L;
+  if (regen) return launch_mag(c, m, nspan, st);
   const int e = time_begin(c, st);
   const hipError_t err = launch_mag(c, m, nspan, st);
   time_end(c, st, e, CLS_MAG, L);
   return err;
+}
+
+// Pairs (bit L / 2) of the speculative batch L = 0, 2, ... <= upto that do
+// not store their first iterate: all but the one whose first launch the last
+// count makes final.
+unsigned skip_mask_for(const amx_fv_ctx* c, int upto) {
+  unsigned m = 0u;
+  for (int L = 0; L <= upto && L <= 62; L += 2)
+    if (L != c->last_k) m |= 1u << (L >> 1);
+  return m;
 }
 
 // ---- one step of slots [0, nspan), dt already in ctl[s].dt, ctl[s].run set.
@@ -2401,8 +2440,13 @@ amx_status step_phase1(amx_fv_ctx* c, const Bufs& b, int nspan, hipStream_t st, 
   int upto = c->last_k < 1 ? 1 : c->last_k;
   if (upto > maxit) upto = maxit;
   *klast = -1;
+  // A pair's first iterate is needed only if it is the final one (the next
+  // pair reads the second): the pairs store it only where the last count
+  // says so, and step_phase3 regenerates it in the steps where it is final
+  // after all (skip_mask)
+  c->skip_mask = skip_mask_for(c, upto);
   for (int L = 0; L <= upto; L += 2) {
-    HIPCHK(launch_mag_pair(c, b, L, nspan, st));
+    HIPCHK(launch_mag_pair(c, b, L, nspan, st, L > 62 || !((c->skip_mask >> (L >> 1)) & 1u)));
     *klast = L + 1;
   }
   return AMX_OK;
@@ -2463,6 +2507,8 @@ amx_status fixed_point_more(amx_fv_ctx* c, const Bufs& b, int nspan, hipStream_t
 amx_status step_phase3(amx_fv_ctx* c, const Bufs& b, int nspan, hipStream_t st, int want_cfl, int spec) {
   if (!c->cfg.mhd) return AMX_OK;
   const int NV = c->NV;
+  // the final iterate, where it is the first of a pair that did not store it
+  if (c->skip_mask) HIPCHK(launch_mag_pair(c, b, 0, nspan, st, 1, 1));
   hipLaunchKernelGGL(k_mag_final, dim3(nspan), dim3(1), 0, st, c->ctl, c->cfg.fixed_point_tol,
                      c->cfg.max_fixed_point_iterations, spec);
   HIPCHK(hipGetLastError());
@@ -3265,14 +3311,18 @@ extern "C" amx_status amx_fv_pool_step_launch(amx_fv_pool* p, void* stream) {
     int upto = c->last_k < 1 ? 1 : c->last_k;
     if (upto > maxit) upto = maxit;
     klast = (upto / 2) * 2 + 1;
+    c->skip_mask = skip_mask_for(c, upto);  // graph replays do not run step_phase1 on the host
+  } else {
+    c->skip_mask = 0u;
   }
+  const int lk = c->cfg.mhd ? c->last_k : 0;  // fixes the launched pairs and which store their first iterate
   if (c->batch > 1) {
     // c->batch whole steps queued at once, each with its second half behind
     // its speculative launches; the device decides per slot whether they
     // settled the fixed point (a slot whose step stalls does nothing until
     // step_finish has completed that step)
     const int nb = c->batch;
-    const auto bkey = std::make_tuple(span, klast, nb);
+    const auto bkey = std::make_tuple(span, lk, nb);
     auto bit = c->g_batch.find(bkey);
     hipGraphExec_t bx = bit == c->g_batch.end() ? nullptr : bit->second;
     amx_status bs = replay_or_capture(c, st, &bx, bx != nullptr, [&]() -> amx_status {
@@ -3300,7 +3350,7 @@ extern "C" amx_status amx_fv_pool_step_launch(amx_fv_pool* p, void* stream) {
     c->batching = 1;
     return AMX_OK;
   }
-  const auto key = std::make_pair(span, klast);
+  const auto key = std::make_pair(span, lk);
   auto it = c->g_launch.find(key);
   hipGraphExec_t x = it == c->g_launch.end() ? nullptr : it->second;
   amx_status s = replay_or_capture(c, st, &x, x != nullptr, [&]() -> amx_status {
@@ -3364,7 +3414,8 @@ extern "C" amx_status amx_fv_pool_step_finish(amx_fv_pool* p, void* stream) {
     c->last_k = kmax;
 
   }
-  auto it = c->g_finish.find(span);
+  const auto fkey = std::make_pair(span, c->skip_mask);
+  auto it = c->g_finish.find(fkey);
   hipGraphExec_t x = it == c->g_finish.end() ? nullptr : it->second;
   amx_status s = replay_or_capture(c, st, &x, x != nullptr, [&]() -> amx_status {
     amx_status r = step_phase3(c, b, span, st, !c->sp.fixed, 0);
@@ -3374,7 +3425,7 @@ extern "C" amx_status amx_fv_pool_step_finish(amx_fv_pool* p, void* stream) {
     return AMX_OK;
   });
   if (s != AMX_OK) return s;
-  if (x) c->g_finish[span] = x;
+  if (x) c->g_finish[fkey] = x;
   HIPCHK(hipMemcpyAsync(c->hctl, c->ctl, sizeof(Ctl) * span, hipMemcpyDeviceToHost, st));
   HIPCHK(hipEventRecord(c->ev_status, st));
   for (int i = 0; i < span; ++i) c->fresh[i] = 0;
