@@ -391,6 +391,10 @@ __device__ __forceinline__ float ld32(const float* base, unsigned c) {
 }
 __device__ __forceinline__ void st32(float* base, unsigned c, float v) { *(float*)((char*)base + (c << 2)) = v; }
 
+// s_waitcnt immediate (gfx9 encoding): vmcnt 0, expcnt and lgkmcnt at their
+// maxima (no wait on them)
+constexpr int VMCNT0 = 0x0F70;
+
 template <int DIM, int MODE>
 struct Raw {
   static constexpr int N = MODE != SRC_MAGFIX ? DIM + 2 : 8;
@@ -625,19 +629,24 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
     x0 = icx * a.lx;
     x1 = min(x0 + a.lx, a.nx);
   }
-  float q0[NV], q1[NV], d0[NV], Fp[NV];
+  // Plane queues of the thread's column, as register rings that the x loop,
+  // unrolled by four, renames instead of copying: q (4 planes: x .. x+2
+  // derived, x+3 in flight), slopes d and x-fluxes F (2 each), the halo
+  // cell's and the Heun centre's raw loads (2 each: plane x, plane x+1 in
+  // flight).
+  float qa[NV], qb[NV], qc[NV], qd[NV], da[NV], db[NV], Fa[NV], Fb[NV];
   if (owner) {
     if (DIM == 3) {
       float qm2[NV], qm1[NV], wl[NV], wr[NV];
       bool f2, f1, fp1;
       load_w<DIM, SRC>(src, xplane(x0 - 2, f2) + col, qm2);
       load_w<DIM, SRC>(src, xplane(x0 - 1, f1) + col, qm1);
-      load_w<DIM, SRC>(src, (unsigned)x0 * sx + col, q0);
-      load_w<DIM, SRC>(src, xplane(x0 + 1, fp1) + col, q1);
+      load_w<DIM, SRC>(src, (unsigned)x0 * sx + col, qa);
+      load_w<DIM, SRC>(src, xplane(x0 + 1, fp1) + col, qb);
       if (BCX) {
         if (f2) qm2[1] = -qm2[1];
         if (f1) qm1[1] = -qm1[1];
-        if (fp1) q1[1] = -q1[1];
+        if (fp1) qb[1] = -qb[1];
       }
       float fa0 = 1.f, fk0 = 1.f, fb0 = 1.f, fam = 1.f, fkm = 1.f, fbm = 1.f;
       if (pp) {
@@ -646,62 +655,88 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
       }
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
-        d0[v] = slope<LIMX>(qm1[v], q0[v], q1[v], lim, ldx);
-        float sm = slope<LIMX>(qm2[v], qm1[v], q0[v], lim, ldx);
+        da[v] = slope<LIMX>(qm1[v], qa[v], qb[v], lim, ldx);
+        float sm = slope<LIMX>(qm2[v], qm1[v], qa[v], lim, ldx);
         if (pp) {
-          d0[v] = d0[v] * fsel(fa0, fk0, fb0, v);
+          da[v] = da[v] * fsel(fa0, fk0, fb0, v);
           sm = sm * fsel(fam, fkm, fbm, v);
         }
         wl[v] = qm1[v] + sm;
-        wr[v] = q0[v] - d0[v];
+        wr[v] = qa[v] - da[v];
       }
-      riemann<DIM, 1, RS>(wl, wr, Fp, ph, alf[0], AMS && gsw[(unsigned)x0 * sx + col]);
+      riemann<DIM, 1, RS>(wl, wr, Fa, ph, alf[0], AMS && gsw[(unsigned)x0 * sx + col]);
     } else {
-      load_w<DIM, SRC>(src, col, q0);
+      load_w<DIM, SRC>(src, col, qa);
     }
   }
 
   // loads for plane x are issued one plane ahead of their use (raw fields in
-  // registers across the previous plane's flux phase)
-  Raw<DIM, SRC> rq, rh;  // own column at x+2, halo cell at x
-  Raw<DIM, W0M> r0;      // Heun centre at x
+  // registers across the previous plane's flux phase). SRC_PLAIN planes are
+  // loaded straight into their q slot; the other forms are derived from a
+  // raw prefetch (rq) when the plane becomes x+2.
+  constexpr bool QRAW = SRC != SRC_PLAIN;
+  Raw<DIM, SRC> rq, ha, hb;  // own column at x+2 (QRAW), halo cell at x / x+1
+  Raw<DIM, W0M> ra, rb;      // Heun centre at x / x+1
   bool fq;  // unused: the flip of a prefetched plane is recomputed when it is derived
-  if (owner && DIM == 3) load_raw<DIM, SRC>(src, xplane(x0 + 2, fq) + col, rq);
-  if (h >= 0) load_raw<DIM, SRC>(src, (DIM == 3 ? (unsigned)x0 * sx : 0u) + hoff, rh);
+  if (owner && DIM == 3) {
+    if (QRAW) {
+      load_raw<DIM, SRC>(src, xplane(x0 + 2, fq) + col, rq);
+    } else {
+      load_w<DIM, SRC>(src, xplane(x0 + 2, fq) + col, qc);
+    }
+  }
+  if (h >= 0) load_raw<DIM, SRC>(src, (DIM == 3 ? (unsigned)x0 * sx : 0u) + hoff, ha);
   // the 8-field SRC_MAGFIX centre would not be prefetched (register pressure):
   // the Heun stage after the magnetic sub-step takes the 5-field SRC_PFIX form
   constexpr bool PRE0 = HEUN && W0M != SRC_MAGFIX;
-  if (PRE0 && active) load_raw<DIM, W0M>(w0p, (DIM == 3 ? (unsigned)x0 * sx : 0u) + col, r0);
+  if (PRE0 && active) load_raw<DIM, W0M>(w0p, (DIM == 3 ? (unsigned)x0 * sx : 0u) + col, ra);
 
   float mx[3] = {0.f, 0.f, 0.f};
-  for (int x = x0; x < x1; ++x) {
+  // plane x: q0..q2 = planes x..x+2 (q2 loaded, or derived here from rq),
+  // q3 receives plane x+3; d0 / Fp: slope and x-flux of plane x (from the
+  // previous plane), d1 / Fn: those of plane x+1; rh0 / r00: halo and Heun
+  // centre of plane x, rh1 / r01 receive plane x+1
+  auto plane_step = [&](const int x, float (&q0)[NV], float (&q1)[NV], float (&q2)[NV], float (&q3)[NV],
+                        float (&d0)[NV], float (&d1)[NV], float (&Fp)[NV], float (&Fn)[NV], Raw<DIM, SRC>& rh0,
+                        Raw<DIM, SRC>& rh1, Raw<DIM, W0M>& r00, Raw<DIM, W0M>& r01) {
     const unsigned plane = DIM == 3 ? (unsigned)x * sx : 0u;
     const unsigned nplane = DIM == 3 ? (unsigned)(x + 1) * sx : 0u;
     const bool more = x + 1 < x1;
-    float Fn[NV], q2[NV], hw[NV];
-    Raw<DIM, W0M> r0c = r0;
+    // The loads issued one plane ago (q2, rh0, r00) have had the previous
+    // plane's flux phase to land: wait for them here, before this plane's
+    // prefetch is issued. Without it, a use of q2 / rh0 / r00 behind the new
+    // loads would wait for those too (one in-order count of vector memory
+    // operations), exposing their latency.
+    if (DIM == 3) __builtin_amdgcn_s_waitcnt(VMCNT0);
+    float hw[NV];
     if (owner && DIM == 3) {
-      derive_w<DIM, SRC>(rq, q2);
+      if (QRAW) derive_w<DIM, SRC>(rq, q2);
       if (BCX) {
         xplane(x + 2, fq);
         if (fq) q2[1] = -q2[1];
       }
     }
     if (h >= 0) {
-      derive_w<DIM, SRC>(rh, hw);
+      derive_w<DIM, SRC>(rh0, hw);
       if (BCX) {
         if (hfa) hw[AXA] = -hw[AXA];
         if (hfb) hw[AXB] = -hw[AXB];
       }
     }
     if (more) {
-      if (owner && DIM == 3) load_raw<DIM, SRC>(src, xplane(x + 3, fq) + col, rq);
-      if (h >= 0) load_raw<DIM, SRC>(src, nplane + hoff, rh);
-      if (PRE0 && active) load_raw<DIM, W0M>(w0p, nplane + col, r0);
+      if (owner && DIM == 3) {
+        if (QRAW) {
+          load_raw<DIM, SRC>(src, xplane(x + 3, fq) + col, rq);
+        } else {
+          load_w<DIM, SRC>(src, xplane(x + 3, fq) + col, q3);
+        }
+      }
+      if (h >= 0) load_raw<DIM, SRC>(src, nplane + hoff, rh1);
+      if (PRE0 && active) load_raw<DIM, W0M>(w0p, nplane + col, r01);
     }
     if (owner) {
       if (DIM == 3) {
-        float d1[NV], wl[NV], wr[NV];
+        float wl[NV], wr[NV];
         float fa1 = 1.f, fk1 = 1.f, fb1 = 1.f;
         if (pp) ppx(x + 1, fa1, fk1, fb1);
 #pragma unroll
@@ -710,7 +745,6 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
           if (pp) d1[v] = d1[v] * fsel(fa1, fk1, fb1, v);
           wl[v] = q0[v] + d0[v];
           wr[v] = q1[v] - d1[v];
-          d0[v] = d1[v];
         }
         riemann<DIM, 1, RS>(wl, wr, Fn, ph, alf[0],
                             AMS && gsw[(unsigned)wrapi(x + 1, a.nx) * sx + col]);
@@ -796,7 +830,7 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
       if (HEUN) {
         float W0[NV], U0[NV], U2[NV];
         if (PRE0) {
-          derive_w<DIM, W0M>(r0c, W0);
+          derive_w<DIM, W0M>(r00, W0);
         } else {
           load_w<DIM, W0M>(w0p, cell, W0);
         }
@@ -822,14 +856,16 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
         for (int k = 0; k < 3; ++k) st32(a.bdst[k] + so, cell, ld32(a.selb[par][k] + so, cell));
       }
     }
-    if (DIM == 3 && owner) {
-#pragma unroll
-      for (int v = 0; v < NV; ++v) {
-        q0[v] = q1[v];
-        q1[v] = q2[v];
-        Fp[v] = Fn[v];
-      }
+  };
+  if (DIM == 3) {
+    for (int x = x0; x < x1; x += 4) {
+      plane_step(x, qa, qb, qc, qd, da, db, Fa, Fb, ha, hb, ra, rb);
+      if (x + 1 < x1) plane_step(x + 1, qb, qc, qd, qa, db, da, Fb, Fa, hb, ha, rb, ra);
+      if (x + 2 < x1) plane_step(x + 2, qc, qd, qa, qb, da, db, Fa, Fb, ha, hb, ra, rb);
+      if (x + 3 < x1) plane_step(x + 3, qd, qa, qb, qc, db, da, Fb, Fa, hb, ha, rb, ra);
     }
+  } else {
+    plane_step(0, qa, qb, qc, qd, da, db, Fa, Fb, ha, hb, ra, rb);
   }
   if (a.want_cfl) {
     __shared__ unsigned sred[DIM][TA + 1];

@@ -33,14 +33,16 @@ if HERE not in sys.path:
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E datasheet peak
 # Algorithmic bytes per cell of one launch (float32 fields read + written once):
 #   k_mag   : one launch makes two fixed-point iterates; the first pair reads
-#             B0, v0, rho (7 fields: the input iterate is B0, v0) and writes both
-#             iterates (12) -> 19 fields; later pairs also read the input iterate
-#             (6) -> 25 fields
+#             B0, v0, rho (7 fields: the input iterate is B0, v0), later pairs
+#             also the input iterate (6). A pair writes its second iterate (6)
+#             and its first only where the previous step's final launch index
+#             says that one is final (never in the blast's steady K = 3) ->
+#             13 fields for the first pair, 19 for later ones
 #   k_stage : mean over the four launches of a step
 #             A1 (R5 W5) + A2 (R10 W5) + A1' (R8 W6: + the fixed pressure)
 #             + A2' (R10 W5: stencil 5, Heun centre 5; the loop keeps B in the
 #             fixed point's iterate sets, no per-step copy) = 54 fields -> 54 B
-MAG_FIELDS_FIRST, MAG_FIELDS_LATER = 19.0, 25.0
+MAG_FIELDS_FIRST, MAG_FIELDS_LATER = 13.0, 19.0
 STAGE_BYTES_PER_CELL = 54.0
 STEP_FIELDS_STAGES = 54
 

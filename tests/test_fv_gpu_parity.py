@@ -319,3 +319,37 @@ def test_hydro_loop_batches_equal_single_steps(gpu, max_steps):
     for _ in range(n):
         t = np.float32(t + np.float32(t_end) / np.float32(steps))
     assert st.t == float(t)
+
+
+@pytest.mark.parametrize("shape", [(16, 16, 16), (64, 64)], ids=["3d", "2d"])
+def test_final_first_iterate_regenerated(gpu, oracle, shape):
+    """A fixed-point pair stores its first iterate only where the previous
+    step's final launch index says it will be the final one; when a step
+    ends on the first launch of a pair that did not store it, a further
+    launch recomputes that iterate on the device. Steps of changing dt on one
+    context drive the final index up and down (5, 2, 4, 0, 6, 2, 3 in 3D)
+    so that both happen, including the final launch 0, whose input is the
+    state's own B; every step must match the oracle's step from the same
+    input, with the same final launch index."""
+    import torch
+    print(f"[gpu] regenerated first iterate {shape}", flush=True)
+    dim = len(shape)
+    state = W.random_smooth(shape, True, seed=8)
+    cfg = W.fv_config(dim, shape[-1], mhd=True, riemann=HLL)
+    nc = native_config(state.shape, cfg, 5.0 / 3.0, 0.4)
+    ctx = _native.FvContext(_native.AmxFvConfig.from_buffer_copy(nc))
+    oc = oracle.make_config(state.shape, dim, C_cfl=0.4)
+    s = torch.from_numpy(np.ascontiguousarray(state)).cuda()
+    prev, regenerated = 4, []  # a fresh context queues launches 0..4 first
+    for dt in [6e-3, 3e-4, 4e-3, 1e-7, 8e-3, 3e-4, 1e-3]:
+        x = s.cpu().numpy()
+        k = ctx.evolve_state(s, torch.tensor(dt, dtype=torch.float32, device="cuda"))
+        torch.cuda.synchronize()
+        ref, kref = oracle.evolve_state_fv(x, oc, np.float32(dt))
+        assert k == kref, (dt, k, kref)
+        err = rel_l1(s.cpu().numpy(), ref)
+        assert err.max() <= TOL, (dt, k, err)
+        if k % 2 == 0 and k != prev and k <= max(1, prev):
+            regenerated.append(k)
+        prev = k
+    assert 0 in regenerated and 2 in regenerated, regenerated
