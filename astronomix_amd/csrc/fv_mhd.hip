@@ -1130,18 +1130,27 @@ struct MagRaw {
   float Bk[3], vk[3], B0[3], v0[3];
 };
 
+// FIRST: the pair of launch 0, whose input iterate is (B0, v0) itself: the
+// inputs are loaded once and stand for the iterate too. VZ: both z velocity
+// rows exist (3D); otherwise a missing row (the state's in 2D) reads as 0.
+template <bool FIRST, bool VZ>
 __device__ __forceinline__ void mag_load(const MagArgs& a, unsigned c, MagRaw& m) {
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     m.B0[i] = ld32(a.B0[i], c);
-    m.Bk[i] = ld32(a.Bk[i], c);
+    m.Bk[i] = FIRST ? m.B0[i] : ld32(a.Bk[i], c);
   }
   m.v0[0] = ld32(a.v0[0], c);
   m.v0[1] = ld32(a.v0[1], c);
-  m.v0[2] = a.v0[2] ? ld32(a.v0[2], c) : 0.f;
-  m.vk[0] = ld32(a.vk[0], c);
-  m.vk[1] = ld32(a.vk[1], c);
-  m.vk[2] = a.vk[2] ? ld32(a.vk[2], c) : 0.f;
+  m.v0[2] = (VZ || a.v0[2]) ? ld32(a.v0[2], c) : 0.f;
+  if (FIRST) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) m.vk[i] = m.v0[i];
+  } else {
+    m.vk[0] = ld32(a.vk[0], c);
+    m.vk[1] = ld32(a.vk[1], c);
+    m.vk[2] = (VZ || a.vk[2]) ? ld32(a.vk[2], c) : 0.f;
+  }
 }
 
 // (avgB, E = avgB x avgv) of iterate (Bk, vk) (_magnetic_field_update.py:84,149-154; _vector_maths.py:5-10)
@@ -1259,7 +1268,9 @@ __device__ __forceinline__ float mag_resid(const float* Bk, const float* vk, con
 // refreshed copy of the edge cell's first iterate, written into the ring by
 // the edge cell's thread (the ghost's own thread, whose stencil would reach
 // past the refreshed layer, writes nothing).
-template <int DIM, int TA, int TBM, bool BCX>
+//
+// FIRST: instantiated for launch 0 only (not for regeneration launches).
+template <int DIM, int TA, int TBM, bool BCX, bool FIRST>
 __global__ __launch_bounds__(((TA + 4) * (TBM + 4) + 255) / 256 * 256) void k_mag(const MagArgs a) {
   constexpr int WX = TBM + 4, NC = (TA + 4) * WX;   // ring-2 tile
   constexpr int W1 = TBM + 2, NC1 = (TA + 2) * W1;  // ring-1 tile
@@ -1380,7 +1391,7 @@ __global__ __launch_bounds__(((TA + 4) * (TBM + 4) + 255) / 256 * 256) void k_ma
     MagRaw rc;
     float d[6], rho = 1.f;
     if (has) {
-      mag_load(m, col, rc);
+      mag_load<FIRST, DIM == 3>(m, col, rc);
       flips(rc, false);
       mag_derive(rc, a.euler, d);
 #pragma unroll
@@ -1439,13 +1450,13 @@ __global__ __launch_bounds__(((TA + 4) * (TBM + 4) + 255) / 256 * 256) void k_ma
     if (has) {
       MagRaw t;
       bool f2, f1;
-      mag_load(m, xoff(x0 - 2, f2) + col, t);
+      mag_load<FIRST, DIM == 3>(m, xoff(x0 - 2, f2) + col, t);
       flips(t, f2);
       mag_derive(t, a.euler, dm);
-      mag_load(m, xoff(x0 - 1, f1) + col, rc);
+      mag_load<FIRST, DIM == 3>(m, xoff(x0 - 1, f1) + col, rc);
       flips(rc, f1);
       mag_derive(rc, a.euler, dc);
-      mag_load(m, (unsigned)x0 * sx + col, rn);
+      mag_load<FIRST, DIM == 3>(m, (unsigned)x0 * sx + col, rn);
       flips(rn, false);
       mag_derive(rn, a.euler, dp);
     }
@@ -1468,7 +1479,7 @@ __global__ __launch_bounds__(((TA + 4) * (TBM + 4) + 255) / 256 * 256) void k_ma
       float rho_nn = 1.f;
       bool fnn;  // flips are applied when the prefetched plane is derived
       const unsigned pnn = xoff(x + 2, fnn) + col;
-      if (has) mag_load(m, pnn, rnn);
+      if (has) mag_load<FIRST, DIM == 3>(m, pnn, rnn);
       if (inner) rho_nn = ld32(m.rho, pnn);
       if (has) {
 #pragma unroll
@@ -2357,14 +2368,28 @@ hipError_t launch_mag(const amx_fv_ctx* c, const MagArgs& m, int nspan, hipStrea
   constexpr int NT = ((TA_MAG + 4) * (TB_MAG + 4) + 255) / 256 * 256;
   static_assert(NT <= 1024, "pair-kernel block too large");
   const dim3 grid(c->ntb_mag * c->nta_mag * c->ncx_mag, nspan), block(NT);
-  if (c->cfg.dim == 3 && c->bcx)
-    hipLaunchKernelGGL((k_mag<3, TA_MAG, TB_MAG, true>), grid, block, 0, st, m);
-  else if (c->cfg.dim == 3)
-    hipLaunchKernelGGL((k_mag<3, TA_MAG, TB_MAG, false>), grid, block, 0, st, m);
-  else if (c->bcx)
-    hipLaunchKernelGGL((k_mag<2, TA_MAG, TB_MAG, true>), grid, block, 0, st, m);
-  else
-    hipLaunchKernelGGL((k_mag<2, TA_MAG, TB_MAG, false>), grid, block, 0, st, m);
+  const bool first = m.L == 0 && !m.regen;
+  if (c->cfg.dim == 3 && c->bcx) {
+    if (first)
+      hipLaunchKernelGGL((k_mag<3, TA_MAG, TB_MAG, true, true>), grid, block, 0, st, m);
+    else
+      hipLaunchKernelGGL((k_mag<3, TA_MAG, TB_MAG, true, false>), grid, block, 0, st, m);
+  } else if (c->cfg.dim == 3) {
+    if (first)
+      hipLaunchKernelGGL((k_mag<3, TA_MAG, TB_MAG, false, true>), grid, block, 0, st, m);
+    else
+      hipLaunchKernelGGL((k_mag<3, TA_MAG, TB_MAG, false, false>), grid, block, 0, st, m);
+  } else if (c->bcx) {
+    if (first)
+      hipLaunchKernelGGL((k_mag<2, TA_MAG, TB_MAG, true, true>), grid, block, 0, st, m);
+    else
+      hipLaunchKernelGGL((k_mag<2, TA_MAG, TB_MAG, true, false>), grid, block, 0, st, m);
+  } else {
+    if (first)
+      hipLaunchKernelGGL((k_mag<2, TA_MAG, TB_MAG, false, true>), grid, block, 0, st, m);
+    else
+      hipLaunchKernelGGL((k_mag<2, TA_MAG, TB_MAG, false, false>), grid, block, 0, st, m);
+  }
   return hipGetLastError();
 }
 
