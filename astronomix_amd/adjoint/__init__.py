@@ -1,4 +1,4 @@
-"""Reverse mode of the finite-volume hydro step loop.
+"""Reverse mode of the finite-volume step loop (hydro and MHD).
 
 ``rollout`` is a ``torch.autograd.Function``: forward runs the step loop with
 the HIP kernels (``amx_fv_evolve_state``) and keeps a checkpoint every
@@ -13,8 +13,12 @@ reference stop_gradients dt (time_integration.py:632-655); with
 (time_integration.py:786-801), whose checkpointing is equinox's
 ``checkpointed_while_loop``.
 
-Scope: hydro (mhd = False) on the implemented finite-volume path; the
-magnetic fixed point has no reverse mode here.
+Scope: the finite-volume path with HLL / HLLC, minmod and periodic
+boundaries, hydro or MHD. For MHD the reverse step differentiates the
+fixed-point applications the forward step made (their number is the
+forward's own: the reverse kernel recomputes the step with the forward
+kernels), as jax.grad through the reference's while loop does
+(_magnetic_field_update.py:171-174).
 """
 
 from __future__ import annotations
@@ -100,14 +104,12 @@ def step_sizes(state, ctx, t_end: float, num_steps: int = None, exact_end: bool 
 
 
 def rollout(state, config, params, registered_variables=None, checkpoint_every: int = 50):
-    """Differentiable time_integration of a hydro state (float32 GPU tensor):
-    returns the final state; gradients flow to `state` through the
+    """Differentiable time_integration of a hydro or MHD state (float32 GPU
+    tensor): returns the final state; gradients flow to `state` through the
     reverse-mode kernels."""
     global _ROLLOUT
     import torch
     require_supported(config, registered_variables)
-    if config.mhd:
-        raise NotImplementedError("reverse mode covers the hydro step only (the magnetic fixed point has none)")
     if config.riemann_solver not in (HLL, HLLC) or config.limiter != MINMOD:
         raise NotImplementedError("reverse mode covers the HLL / HLLC fluxes with the minmod limiter")
     if any(boundary_codes(config)):

@@ -1877,6 +1877,8 @@ struct amx_fv_ctx {
   int stepping;     // pool: a step is launched and not finished
   hipEvent_t ev_status;
   float* adj;       // reverse-mode workspace (allocated on first use)
+  float* adjm;      // MHD reverse mode: state copy, sub-step buffers and the fixed point's iterates
+  int adjm_iters;   // iterates adjm holds room for
   unsigned char* amg;  // hybrid / AM-HLLC compression switch, one byte per cell and slot
   float* ppf;       // VAN_ALBADA_PP factors: 2 dim + 1 fields per cell and slot
   amx_step_hook hook;  // between-step hook of the step loop (forcing), or null
@@ -2042,6 +2044,7 @@ void free_ctx(amx_fv_ctx* c) {
   (void)hipDeviceSynchronize();
   if (c->mem) (void)hipFree(c->mem);
   if (c->adj) (void)hipFree(c->adj);
+  if (c->adjm) (void)hipFree(c->adjm);
   if (c->amg) (void)hipFree(c->amg);
   if (c->ppf) (void)hipFree(c->ppf);
   if (c->snapwork) (void)hipFree(c->snapwork);
@@ -3188,11 +3191,164 @@ amx_status adjoint_step_t(amx_fv_ctx* c, const float* w0, const float* lam_out, 
 
 }  // namespace
 
+// Fields of the MHD reverse mode's workspace before the iterates: state copy
+// (NS), W_B (NV), lam of W_B (NV), lam of W_A (NV), two 6-row cotangents, X
+// and Y (3 + 3), the accumulated cotangents of B0, v0, rho (7).
+long adjm_fixed_fields(const amx_fv_ctx* c) { return (long)c->NS + 3L * c->NV + 12 + 6 + 7; }
+
+amx_status adjm_reserve(amx_fv_ctx* c, int iters, hipStream_t st) {
+  if (c->adjm && iters <= c->adjm_iters) return AMX_OK;
+  int cap = c->adjm_iters > 0 ? c->adjm_iters : 8;
+  while (cap < iters) cap *= 2;
+  const size_t bytes = sizeof(float) * (size_t)(adjm_fixed_fields(c) + 6L * cap) * c->ncell;
+  float* m = nullptr;
+  if (hipMalloc((void**)&m, bytes) != hipSuccess) return fail(AMX_ERR_NOMEM, "reverse-mode workspace allocation failed%s");
+  if (c->adjm) {
+    HIPCHK(hipStreamSynchronize(st));
+    HIPCHK(hipMemcpy(m, c->adjm, sizeof(float) * (size_t)(adjm_fixed_fields(c) + 6L * c->adjm_iters) * c->ncell,
+                     hipMemcpyDeviceToDevice));
+    (void)hipFree(c->adjm);
+  }
+  c->adjm = m;
+  c->adjm_iters = cap;
+  return AMX_OK;
+}
+
+// Reverse mode of one MHD step S_A(dt/2) S_B(dt) S_A(dt/2)
+// (evolve_state.py:496-600): the forward pass is recomputed from a copy of
+// the state with the forward kernels (so the stage states, the fixed-point
+// iterates and their count are the forward's own), keeping every iterate;
+// then the second gas half, the magnetic sub-step and the first gas half
+// are reversed (adj::k_adjm_*, adjoint_step_t at dt/2).
+template <int DIM>
+amx_status adjoint_step_mhd(amx_fv_ctx* c, const float* state, const float* lam_out, float* lam_in,
+                            hipStream_t st) {
+  constexpr int NV = DIM + 2;
+  const long n = c->ncell;
+  amx_status r = adjm_reserve(c, 8, st);
+  if (r != AMX_OK) return r;
+  HIPCHK(hipMemcpyAsync(c->adjm, state, sizeof(float) * (size_t)c->NS * n, hipMemcpyDeviceToDevice, st));
+  Bufs b = bufs(c, c->adjm);
+  // ---- forward: S_A(dt/2) into S's gas rows (as step_phase1 does)
+  StageArgs a1 = stage_base(c, 0.5f);
+  for (int v = 0; v < NV; ++v) {
+    a1.src[v] = b.gas[v];
+    a1.out[v] = b.G1 + (long)v * n;
+  }
+  HIPCHK((launch_stage<SRC_PLAIN, false, SRC_PLAIN>(c, a1, 1, st)));
+  StageArgs a2 = stage_base(c, 0.5f);
+  for (int v = 0; v < NV; ++v) {
+    a2.src[v] = b.G1 + (long)v * n;
+    a2.w0[v] = b.gas[v];
+    a2.out[v] = b.gas[v];
+  }
+  HIPCHK((launch_stage<SRC_PLAIN, true, SRC_PLAIN>(c, a2, 1, st)));
+  // ---- forward: the fixed point pair by pair, every iterate kept (launch k's
+  // output at iterate slot k), until the residuals settle it
+  c->skip_mask = 0u;
+  int F = -1;
+  for (int L = 0; F < 0; L += 2) {
+    r = adjm_reserve(c, L + 2, st);  // may move the workspace: the buffers are taken again
+    if (r != AMX_OK) return r;
+    b = bufs(c, c->adjm);
+    float* IT = c->adjm + adjm_fixed_fields(c) * n;
+    HIPCHK(launch_mag_pair(c, b, L, 1, st, 1, 0));
+    for (int k = L; k <= L + 1; ++k)
+      for (int f = 0; f < 6; ++f)
+        HIPCHK(hipMemcpyAsync(IT + (long)(6 * k + f) * n, b.P[iterate_set(-1, k)][f], sizeof(float) * n,
+                              hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(c->hctl, c->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    F = slot_final(c, c->hctl[0]);
+  }
+  float* S2 = c->adjm;
+  const Bufs bb = bufs(c, S2);
+  float* WB = S2 + (long)c->NS * n;
+  float* LWB = WB + (long)NV * n;
+  float* LWA = LWB + (long)NV * n;
+  float* LB0 = LWA + (long)NV * n;  // two 6-row cotangents
+  float* LB1 = LB0 + 6L * n;
+  float* XY = LB1 + 6L * n;
+  float* ACC = XY + 6L * n;
+  float* IT = ACC + 7L * n;
+  adj::AdjGeom g;
+  if (DIM == 3) {
+    g.n[0] = c->cfg.nx;
+    g.n[1] = c->cfg.ny;
+    g.n[2] = c->cfg.nz;
+    g.st[2] = 1;
+    g.st[1] = c->cfg.nz;
+    g.st[0] = (long)c->cfg.ny * c->cfg.nz;
+  } else {
+    g.n[0] = c->cfg.nx;
+    g.n[1] = c->cfg.ny;
+    g.n[2] = 1;
+    g.st[1] = 1;
+    g.st[0] = c->cfg.ny;
+    g.st[2] = n;
+  }
+  g.nc = n;
+  long blocks = (n + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  const dim3 grid(blocks), blk(256);
+  hipLaunchKernelGGL(adj::k_adjm_wb<DIM>, grid, blk, 0, st, bb.gas[0], IT + 6L * F * n, WB, g);
+  HIPCHK(hipGetLastError());
+  // ---- reverse: second gas half (dt/2) from lam_out's gas rows
+  hipLaunchKernelGGL(adj::k_scale_dt, dim3(1), dim3(1), 0, st, c->ctl, 0.5f);
+  HIPCHK(hipGetLastError());
+  amx_status s = adjoint_step_t<DIM>(c, WB, lam_out, LWB, st);
+  if (s != AMX_OK) return s;
+  hipLaunchKernelGGL(adj::k_scale_dt, dim3(1), dim3(1), 0, st, c->ctl, 2.0f);
+  HIPCHK(hipGetLastError());
+  // ---- reverse: the magnetic sub-step, launch F down to 0
+  hipLaunchKernelGGL(adj::k_adjm_init<DIM>, grid, blk, 0, st, lam_out + (long)NV * n, LWB, LB0, ACC, g);
+  HIPCHK(hipGetLastError());
+  adj::MagAdjArgs m;
+  memset(&m, 0, sizeof(m));
+  for (int k = 0; k < 3; ++k) {
+    m.B0[k] = bb.B[k];
+    m.v0[k] = k < DIM ? bb.gas[1 + k] : nullptr;
+  }
+  m.rho = bb.gas[0];
+  m.X = XY;
+  m.Y = XY + 3L * n;
+  m.acc = ACC;
+  m.ctl = c->ctl;
+  m.inv_dx = c->hph[0].inv_dx;
+  m.euler = c->cfg.magnetic_integrator == AMX_IMPLICIT_EULER;
+  m.g = g;
+  float* lcur = LB0;
+  float* lnext = LB1;
+  for (int j = F; j >= 0; --j) {
+    m.first = j == 0;
+    for (int k = 0; k < 3; ++k) {
+      m.Bj[k] = j == 0 ? bb.B[k] : IT + (long)(6 * (j - 1) + k) * n;
+      m.vj[k] = j == 0 ? m.v0[k] : IT + (long)(6 * (j - 1) + 3 + k) * n;
+    }
+    m.lB = lcur;
+    m.lBn = lnext;
+    hipLaunchKernelGGL(adj::k_adjm_xy<DIM>, grid, blk, 0, st, m);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(adj::k_adjm_back<DIM>, grid, blk, 0, st, m);
+    HIPCHK(hipGetLastError());
+    std::swap(lcur, lnext);
+  }
+  hipLaunchKernelGGL(adj::k_adjm_final<DIM>, grid, blk, 0, st, LWB, lcur, ACC, LWA, lam_in + (long)NV * n, g);
+  HIPCHK(hipGetLastError());
+  // ---- reverse: first gas half (dt/2) at the step's input state
+  hipLaunchKernelGGL(adj::k_scale_dt, dim3(1), dim3(1), 0, st, c->ctl, 0.5f);
+  HIPCHK(hipGetLastError());
+  s = adjoint_step_t<DIM>(c, state, LWA, lam_in, st);
+  if (s != AMX_OK) return s;
+  hipLaunchKernelGGL(adj::k_scale_dt, dim3(1), dim3(1), 0, st, c->ctl, 2.0f);
+  HIPCHK(hipGetLastError());
+  return AMX_OK;
+}
+
 extern "C" amx_status amx_fv_adjoint_step(amx_fv_ctx* c, const float* state, const float* d_dt, const float* lam_out,
                                           float* lam_in, void* stream) {
   if (!c || !state || !d_dt || !lam_out || !lam_in) return fail(AMX_ERR_INVALID, "null argument%s");
   if (c->own_state) return fail(AMX_ERR_INVALID, "a pool context has no reverse mode%s");
-  if (c->cfg.mhd) return fail(AMX_ERR_UNSUPPORTED, "reverse mode covers the hydro step only%s");
   if ((c->cfg.riemann != AMX_HLL && c->cfg.riemann != AMX_HLLC) || c->cfg.limiter != AMX_MINMOD)
     return fail(AMX_ERR_UNSUPPORTED, "reverse mode covers the HLL / HLLC fluxes with the minmod limiter%s");
   if (c->bcx) return fail(AMX_ERR_UNSUPPORTED, "reverse mode covers periodic boundaries only%s");
@@ -3208,6 +3364,8 @@ extern "C" amx_status amx_fv_adjoint_step(amx_fv_ctx* c, const float* state, con
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, st, c->ctl, d_dt);
   HIPCHK(hipGetLastError());
+  if (c->cfg.mhd)
+    return dim == 3 ? adjoint_step_mhd<3>(c, state, lam_out, lam_in, st) : adjoint_step_mhd<2>(c, state, lam_out, lam_in, st);
   return dim == 3 ? adjoint_step_t<3>(c, state, lam_out, lam_in, st) : adjoint_step_t<2>(c, state, lam_out, lam_in, st);
 }
 

@@ -161,16 +161,20 @@ amx_status amx_fv_evolve_state_noalloc(const amx_fv_config* cfg, float* state, c
 amx_status amx_fv_time_integration(amx_fv_ctx* ctx, float* state, const amx_fv_run* run,
                                    amx_fv_stats* stats, void* stream);
 
-/* Reverse mode of one hydro step (mhd = 0): lam_in = (dW_out/dW)^T lam_out
- * where W_out is the state amx_fv_evolve_state makes from `state` with the
- * step size in d_dt, held constant (the reference stop_gradients dt,
+/* Reverse mode of one step: lam_in = (dW_out/dW)^T lam_out where W_out is
+ * the state amx_fv_evolve_state makes from `state` with the step size in
+ * d_dt, held constant (the reference stop_gradients dt,
  * time_integration.py:632-655). `state`, lam_out and lam_in are
  * (num_vars, nx, ny[, nz]) device arrays; `state` is not modified; lam_in
- * must not alias the inputs. The two stage states are recomputed with the
- * forward kernel. Replaces jax.grad through _evolve_gas_state_unsplit
- * (evolve_state.py:418-488) for the differentiation modes of
- * time_integration.py:786-801; the step loop and its checkpointing are the
- * caller's (astronomix_amd.adjoint). */
+ * must not alias the inputs. The forward pass is recomputed with the forward
+ * kernels (the stage states; with mhd = 1 also the magnetic fixed point's
+ * iterates, every one kept, and their count). Replaces jax.grad through
+ * _evolve_gas_state_unsplit (evolve_state.py:418-488) and, with mhd = 1,
+ * through _evolve_state_fv's split step (evolve_state.py:496-600) and the
+ * fixed point's checkpointed while loop (_magnetic_field_update.py:171-174),
+ * for the differentiation modes of time_integration.py:786-801; the step
+ * loop and its checkpointing are the caller's (astronomix_amd.adjoint). HLL /
+ * HLLC with minmod, periodic boundaries. */
 amx_status amx_fv_adjoint_step(amx_fv_ctx* ctx, const float* state, const float* d_dt, const float* lam_out,
                                float* lam_in, void* stream);
 
