@@ -9,6 +9,7 @@ line (rank 0 only; this build runs on one GPU).
     python bench.py [--steps 200] [--warmup 10] [--n 256] [--no-cpu-baseline] [--no-serving]
     python bench.py --workload mhd_turbulence_3d_fd [--steps 100]   # FD WENO-5 + CT
     python bench.py --workload serving_mix                          # the served mix alone
+    python bench.py --workload adjoint_ot                           # MHD reverse mode, OT 256^2
 
 The default line also carries the metric's second half, "served": solves/s and
 p50 / p99 latency of the mixed Orszag-Tang / blast request stream served with
@@ -360,6 +361,78 @@ def bench_adjoint(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_adjoint_mhd(args):
+    """Gradient of L = sum (rho_final - target)^2 with respect to the whole
+    initial state of the 2D Orszag-Tang vortex (256^2, HLL, minmod, split MHD
+    step with the implicit-midpoint magnetic fixed point, 200 fixed steps to
+    t = 0.1, checkpoint every 50 steps): forward run + checkpointed reverse
+    sweep through the adjoint kernels, the fixed point's applications
+    included."""
+    import numpy as np
+    import torch
+    from astronomix_amd import workloads as W
+    from astronomix_amd.adjoint import rollout
+    from astronomix_amd.option_classes.simulation_config import HLL
+    from astronomix_amd.option_classes.simulation_params import SimulationParams
+
+    n, steps = args.n, args.steps
+    cfg = W.fv_config(2, n, mhd=True, riemann=HLL)._replace(fixed_timestep=True, num_timesteps=steps)
+    params = SimulationParams(C_cfl=0.4, t_end=0.1 * steps / 200, gamma=5.0 / 3.0)
+    s0 = torch.from_numpy(W.orszag_tang_2d(n)).cuda()
+    target = torch.from_numpy(np.random.default_rng(1).random((n, n)).astype(np.float32)).cuda()
+
+    def grad_state():
+        w = s0.clone().requires_grad_(True)
+        out = rollout(w, cfg, params, checkpoint_every=50)
+        loss = ((out[0].double() - target.double()) ** 2).sum()
+        loss.backward()
+        return loss.detach(), w.grad
+
+    if args.warmup > 0:
+        grad_state()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    loss, g = grad_state()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    with torch.no_grad():
+        rollout(s0.clone(), cfg, params, checkpoint_every=50)
+    torch.cuda.synchronize()
+    fwd = time.perf_counter() - t1
+    out = {
+        "metric": "adjoint: zone-updates/s through forward + checkpointed reverse sweep, 2D Orszag-Tang MHD HLL",
+        "value": round(steps * n * n / el / 1e6, 3),
+        "unit": "Mzone-updates/s (forward+reverse)",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(el / steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"adjoint_ot_{n}: grad of sum((rho_T - target)^2) wrt the initial state; Orszag-Tang; HLL + "
+                        f"minmod, split MHD step with the implicit-midpoint fixed point, {steps} fixed steps to "
+                        f"t = {0.1 * steps / 200:g}, checkpoint every 50",
+            "grid": [n, n],
+            "parallelism": "single GPU",
+        },
+        "gradient_seconds": round(el, 4),
+        "forward_seconds": round(fwd, 4),
+        "reverse_over_forward": round((el - fwd) / fwd, 3),
+        "loss": float(loss),
+        "grad_norm": float(g.double().norm()),
+        "grad_finite": bool(torch.isfinite(g).all()),
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def stream_copy_gbs(gib: float = 1.0, reps: int = 10):
     """HBM ceiling on this box: the library's streaming copy kernel on a
     buffer of `gib` GiB (read + write counted), timed with HIP events."""
@@ -392,7 +465,7 @@ def main():
                     help="CPU baseline sample length (default: 8 FV steps at 256^3, 6 FD steps at 128^3)")
     ap.add_argument("--copy-gib", type=float, default=1.0)
     ap.add_argument("--workload", default="mhd_blast_3d",
-                    choices=["mhd_blast_3d", "mhd_turbulence_3d_fd", "serving_mix", "adjoint_kh"])
+                    choices=["mhd_blast_3d", "mhd_turbulence_3d_fd", "serving_mix", "adjoint_kh", "adjoint_ot"])
     ap.add_argument("--requests", type=int, default=32, help="serving_mix: requests in the stream")
     ap.add_argument("--slots", type=int, default=16, help="serving_mix: slots per grid-class pool")
     ap.add_argument("--hbm-budget-gib", type=float, default=32.0, help="serving_mix: pool memory budget")
@@ -416,6 +489,8 @@ def main():
         return bench_serving(args)
     if args.workload == "adjoint_kh":
         return bench_adjoint(args)
+    if args.workload == "adjoint_ot":
+        return bench_adjoint_mhd(args)
 
     import torch
     from astronomix_amd import _native
