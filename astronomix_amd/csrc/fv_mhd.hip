@@ -1683,8 +1683,7 @@ __device__ __forceinline__ void zero_cflp(Ctl* c) {
 
 // One thread per slot (slot = blockIdx.x): the step's dt, and whether the
 // slot runs this step (live and not yet at t_end / num_timesteps).
-__global__ void k_step_begin(Ctl* ctl, StepParams p) {
-  Ctl* const c = ctl + blockIdx.x;
+__device__ void step_begin(Ctl* c, const StepParams& p) {
   if (p.batch_first) c->iters_bmax = 0;
   if (!c->live) {
     c->run = 0;
@@ -1712,6 +1711,8 @@ __global__ void k_step_begin(Ctl* ctl, StepParams p) {
   fixed_point_reset(c);
 }
 
+__global__ void k_step_begin(Ctl* ctl, StepParams p) { step_begin(ctl + blockIdx.x, p); }
+
 // dt given by the caller (one step of _evolve_state_fv)
 __global__ void k_set_dt(Ctl* c, const float* dtp) {
   c->dt = *dtp;
@@ -1725,8 +1726,7 @@ __global__ void k_set_dt(Ctl* c, const float* dtp) {
 
 // keep_b: the final B of the step stays in its iterate set, which holds B0
 // of the next step (the state's B rows are written once, by k_copy_bset).
-__global__ void k_step_end(Ctl* ctl, int keep_b) {
-  Ctl* const c = ctl + blockIdx.x;
+__device__ void step_end(Ctl* c, int keep_b) {
   if (!c->run || c->held) return;  // held: the host finishes the step
   c->t = c->t + c->dt;
   c->steps += 1;
@@ -1734,6 +1734,15 @@ __global__ void k_step_end(Ctl* ctl, int keep_b) {
   c->iters_max = c->iters > c->iters_max ? c->iters : c->iters_max;
   c->iters_bmax = c->iters > c->iters_bmax ? c->iters : c->iters_bmax;
   if (keep_b) c->bset = iterate_set(c->bset, c->iters);
+}
+
+__global__ void k_step_end(Ctl* ctl, int keep_b) { step_end(ctl + blockIdx.x, keep_b); }
+
+// the end of one queued step and the beginning of the next, in one launch
+__global__ void k_step_end_begin(Ctl* ctl, int keep_b, StepParams p) {
+  Ctl* const c = ctl + blockIdx.x;
+  step_end(c, keep_b);
+  step_begin(c, p);
 }
 
 // B of slot 0 from its iterate set into the state's B rows (a loop that kept
@@ -2946,12 +2955,16 @@ amx_status fv_loop(amx_fv_ctx* c, float* state, const amx_fv_run* run, snap::Dri
       long n = BATCH;
       if (run->max_steps >= 0 && run->max_steps - steps < n) n = run->max_steps - steps;
       if (run->fixed_timestep && run->num_timesteps - steps < n) n = run->num_timesteps - steps;
+      hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(1), 0, st, c->ctl, sp);
+      HIPCHK(hipGetLastError());
       for (long i = 0; i < n; ++i) {
-        hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(1), 0, st, c->ctl, sp);
-        HIPCHK(hipGetLastError());
         amx_status s = do_step(c, state, st, !run->fixed_timestep, nullptr);
         if (s != AMX_OK) return s;
-        hipLaunchKernelGGL(k_step_end, dim3(1), dim3(1), 0, st, c->ctl, 0);
+        // this step's end and the next one's beginning
+        if (i + 1 < n)
+          hipLaunchKernelGGL(k_step_end_begin, dim3(1), dim3(1), 0, st, c->ctl, 0, sp);
+        else
+          hipLaunchKernelGGL(k_step_end, dim3(1), dim3(1), 0, st, c->ctl, 0);
         HIPCHK(hipGetLastError());
       }
       HIPCHK(hipMemcpyAsync(c->hctl, c->ctl, offsetof(Ctl, cflp), hipMemcpyDeviceToHost, st));
@@ -2966,16 +2979,21 @@ amx_status fv_loop(amx_fv_ctx* c, float* state, const amx_fv_run* run, snap::Dri
       if (run->max_steps >= 0 && run->max_steps - steps < n) n = run->max_steps - steps;
       if (run->fixed_timestep && run->num_timesteps - steps < n) n = run->num_timesteps - steps;
       int klast = 0;
+      StepParams spi = sp;
+      spi.batch_first = 1;
+      hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(1), 0, st, c->ctl, spi);
+      HIPCHK(hipGetLastError());
+      spi.batch_first = 0;
       for (long i = 0; i < n; ++i) {
-        StepParams spi = sp;
-        spi.batch_first = i == 0;
-        hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(1), 0, st, c->ctl, spi);
-        HIPCHK(hipGetLastError());
         amx_status s = step_phase1(c, bl, 1, st, !run->fixed_timestep, &klast);
         if (s != AMX_OK) return s;
         s = step_phase3(c, bl, 1, st, !run->fixed_timestep, 1);
         if (s != AMX_OK) return s;
-        hipLaunchKernelGGL(k_step_end, dim3(1), dim3(1), 0, st, c->ctl, c->keep_b);
+        // this step's end and the next one's beginning
+        if (i + 1 < n)
+          hipLaunchKernelGGL(k_step_end_begin, dim3(1), dim3(1), 0, st, c->ctl, c->keep_b, spi);
+        else
+          hipLaunchKernelGGL(k_step_end, dim3(1), dim3(1), 0, st, c->ctl, c->keep_b);
         HIPCHK(hipGetLastError());
       }
       HIPCHK(hipMemcpyAsync(c->hctl, c->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
@@ -3546,17 +3564,22 @@ extern "C" amx_status amx_fv_pool_step_launch(amx_fv_pool* p, void* stream) {
     auto bit = c->g_batch.find(bkey);
     hipGraphExec_t bx = bit == c->g_batch.end() ? nullptr : bit->second;
     amx_status bs = replay_or_capture(c, st, &bx, bx != nullptr, [&]() -> amx_status {
+      StepParams spi = c->sp;
+      spi.batch_first = 1;
+      hipLaunchKernelGGL(k_step_begin, dim3(span), dim3(1), 0, st, c->ctl, spi);
+      HIPCHK(hipGetLastError());
+      spi.batch_first = 0;
       for (int i = 0; i < nb; ++i) {
-        StepParams spi = c->sp;
-        spi.batch_first = i == 0;
-        hipLaunchKernelGGL(k_step_begin, dim3(span), dim3(1), 0, st, c->ctl, spi);
-        HIPCHK(hipGetLastError());
         int kl = 0;
         amx_status r = step_phase1(c, b, span, st, !c->sp.fixed, &kl);
         if (r != AMX_OK) return r;
         r = step_phase3(c, b, span, st, !c->sp.fixed, 1);
         if (r != AMX_OK) return r;
-        hipLaunchKernelGGL(k_step_end, dim3(span), dim3(1), 0, st, c->ctl, c->keep_b);
+        // this step's end and the next one's beginning
+        if (i + 1 < nb)
+          hipLaunchKernelGGL(k_step_end_begin, dim3(span), dim3(1), 0, st, c->ctl, c->keep_b, spi);
+        else
+          hipLaunchKernelGGL(k_step_end, dim3(span), dim3(1), 0, st, c->ctl, c->keep_b);
         HIPCHK(hipGetLastError());
       }
       return AMX_OK;
