@@ -87,6 +87,8 @@ struct Ctl {
   unsigned lfp[3][NSLOT];       // Lax-Friedrichs: partial maxima of |u| + c of the stage input
   int bset;                     // iterate set holding B0 of the magnetic sub-step; -1: the state's B rows
   int held;                     // the fixed point had not settled when k_mag_final ran: gated stage kernels return
+  int kprev;                    // final launch index of the slot's previous step (k_mag_final): the speculative
+                                // pair starting there stores its first iterate, the others leave it out
 };
 
 // Iterate set of fixed-point launch k: the three sets other than bset
@@ -1102,9 +1104,9 @@ struct MagArgs {
   long sstride;        // floats between consecutive slots' buffers
   int slot0;
   int L;               // first launch index of the pair (even)
-  int write_first;     // store the pair's first iterate (else only a regeneration would)
+  int write_first;     // store the pair's first iterate (0: only where L is the slot's kprev)
   int regen;           // regeneration launch: see k_mag
-  unsigned skip_mask;  // regeneration: pairs (bit L / 2) queued without storing their first iterate
+  unsigned skip_mask;  // regeneration: speculative pairs (bit L / 2), whose first iterate depends on kprev
   int max_iter;
   float tol;
   int euler;           // IMPLICIT_EULER: average = iterate
@@ -1283,14 +1285,16 @@ __global__ __launch_bounds__(((TA + 4) * (TBM + 4) + 255) / 256 * 256) void k_ma
   if (a.regen) {
     // Regeneration: the fixed point's final launch is read from the residuals
     // of the last executed pair; if it is that pair's first launch and the
-    // pair did not store its iterate, the launch is computed again (the same
+    // pair did not store its iterate (a speculative pair not starting at the
+    // previous step's final index), the launch is computed again (the same
     // arithmetic on the same inputs, so bit-identical) and only that iterate
     // is stored. Residuals, counters and flags are left as they are.
     const int Ll = C->iters - 1;
     if (Ll < 0 || Ll > 62) return;
     const float r0 = __uint_as_float(wave_max_u(C->resp[Ll % NRES][tid & 63]));
     const float r1 = __uint_as_float(wave_max_u(C->resp[(Ll + 1) % NRES][tid & 63]));
-    if (pair_final(Ll, r0, r1, a.tol, a.max_iter) != Ll || !((a.skip_mask >> (Ll >> 1)) & 1u)) return;
+    if (pair_final(Ll, r0, r1, a.tol, a.max_iter) != Ll || !((a.skip_mask >> (Ll >> 1)) & 1u) || Ll == C->kprev)
+      return;
     L = Ll;
   } else if (L > 0) {
     if (C->done) return;
@@ -1309,7 +1313,7 @@ __global__ __launch_bounds__(((TA + 4) * (TBM + 4) + 255) / 256 * 256) void k_ma
     C->resp[(L + 2) % NRES][tid] = 0u;
     C->resp[(L + 3) % NRES][tid] = 0u;
   }
-  const bool store1 = a.write_first || a.regen;
+  const bool store1 = a.write_first || a.regen || L == C->kprev;
   const long so = slot * a.sstride;
   MagArgs m = a;  // slot-local pointers
   // B0 is in the state's rows or, in a loop that keeps B in the iterate
@@ -1597,6 +1601,7 @@ __global__ void k_mag_final(Ctl* ctl, float tol, int max_iter, int spec) {
     return;
   }
   c->iters = f >= 0 ? f : L + 1;
+  if (f >= 0) c->kprev = f;
   c->held = 0;
   c->stall = 0;
 }
@@ -1781,7 +1786,9 @@ __global__ void k_cfl_out(Ctl* c, float C, float dx, int dim, float* out) {
 __global__ void k_zero_cfl(Ctl* c) { zero_cflp(c); }
 
 // (Re)start slot blockIdx.x + slot0 at t = 0 (live = 1), or empty it (live = 0).
-__global__ void k_slot_reset(Ctl* ctl, int slot0, float t_end, int live) {
+// fresh: a new simulation in the slot (a loop restarting on the same
+// simulation keeps kprev, the fixed point's last final launch index)
+__global__ void k_slot_reset(Ctl* ctl, int slot0, float t_end, int live, int fresh) {
   Ctl* const c = ctl + slot0 + blockIdx.x;
   c->dt = 0.f;
   c->t = 0.f;
@@ -1799,6 +1806,7 @@ __global__ void k_slot_reset(Ctl* ctl, int slot0, float t_end, int live) {
   c->bset = -1;
   c->held = 0;
   c->stall = 0;
+  if (fresh) c->kprev = -1;
   c->iters_sum = 0;
   c->iters_max = 0;
   c->iters_bmax = 0;
@@ -2222,7 +2230,7 @@ amx_status make_ctx(const amx_fv_config* cfg, int cap, int own_state, amx_fv_ctx
     return fail(AMX_ERR_HIP, "parameter upload failed%s");
   }
   // single simulation: slot 0 is live from the start; pool slots start empty
-  hipLaunchKernelGGL(k_slot_reset, dim3(cap), dim3(1), 0, 0, c->ctl, 0, 0.f, own_state ? 0 : 1);
+  hipLaunchKernelGGL(k_slot_reset, dim3(cap), dim3(1), 0, 0, c->ctl, 0, 0.f, own_state ? 0 : 1, 1);
   if (hipDeviceSynchronize() != hipSuccess) {
     free_ctx(c);
     return fail(AMX_ERR_HIP, "init sync failed%s");
@@ -2452,13 +2460,13 @@ hipError_t launch_mag_pair(amx_fv_ctx* c, const Bufs& b, int L, int nspan, hipSt
   return err;
 }
 
-// Pairs (bit L / 2) of the speculative batch L = 0, 2, ... <= upto that do
-// not store their first iterate: all but the one whose first launch the last
-// count makes final.
+// Pairs (bit L / 2) of the speculative batch L = 0, 2, ... <= upto whose
+// first iterate is stored only by the pair starting at the slot's previous
+// final launch index (ctl.kprev, decided on the device).
 unsigned skip_mask_for(const amx_fv_ctx* c, int upto) {
+  (void)c;
   unsigned m = 0u;
-  for (int L = 0; L <= upto && L <= 62; L += 2)
-    if (L != c->last_k) m |= 1u << (L >> 1);
+  for (int L = 0; L <= upto && L <= 62; L += 2) m |= 1u << (L >> 1);
   return m;
 }
 
@@ -2515,9 +2523,9 @@ amx_status step_phase1(amx_fv_ctx* c, const Bufs& b, int nspan, hipStream_t st, 
   if (upto > maxit) upto = maxit;
   *klast = -1;
   // A pair's first iterate is needed only if it is the final one (the next
-  // pair reads the second): the pairs store it only where the last count
-  // says so, and step_phase3 regenerates it in the steps where it is final
-  // after all (skip_mask)
+  // pair reads the second): each slot's pairs store it only where its
+  // previous step ended (ctl.kprev), and step_phase3 regenerates it in the
+  // steps where it is final after all (skip_mask: the speculative pairs)
   c->skip_mask = skip_mask_for(c, upto);
   for (int L = 0; L <= upto; L += 2) {
     HIPCHK(launch_mag_pair(c, b, L, nspan, st, L > 62 || !((c->skip_mask >> (L >> 1)) & 1u)));
@@ -2828,6 +2836,7 @@ __global__ void k_noalloc_begin(Ctl* c, Phys* ph, Phys v, const float* dtp) {
   c->dt = *dtp;
   c->run = 1;
   c->bset = -1;
+  c->kprev = -1;
   for (int i = 0; i < 4; ++i) c->cfl[i] = 0u;
   zero_cflp(c);
   fixed_point_reset(c);
@@ -2907,7 +2916,7 @@ amx_diag_desc fv_diag_desc(const amx_fv_ctx* c, double box_size) {
 // snapshot driver, rows are taken as in time_integration.py:539-610, 736-745.
 amx_status fv_loop(amx_fv_ctx* c, float* state, const amx_fv_run* run, snap::Driver* snp, amx_fv_stats* stats,
                    hipStream_t st) {
-  hipLaunchKernelGGL(k_slot_reset, dim3(1), dim3(1), 0, st, c->ctl, 0, run->t_end, 1);
+  hipLaunchKernelGGL(k_slot_reset, dim3(1), dim3(1), 0, st, c->ctl, 0, run->t_end, 1, 0);
   HIPCHK(hipGetLastError());
   // Without snapshot rows or a step hook nothing reads the state's B rows
   // between steps: the final B of each step stays in its iterate set (the
@@ -3488,7 +3497,7 @@ extern "C" amx_status amx_fv_pool_load(amx_fv_pool* p, int slot, const float* st
   HIPCHK(hipStreamSynchronize(st));  // the pinned parameter mirror is rewritten below
   c->hph[slot] = make_phys(gamma, c->cfg.grid_spacing);
   HIPCHK(hipMemcpyAsync(c->dph + slot, c->hph + slot, sizeof(Phys), hipMemcpyHostToDevice, st));
-  hipLaunchKernelGGL(k_slot_reset, dim3(1), dim3(1), 0, st, c->ctl, slot, t_end, 1);
+  hipLaunchKernelGGL(k_slot_reset, dim3(1), dim3(1), 0, st, c->ctl, slot, t_end, 1, 1);
   HIPCHK(hipGetLastError());
   if (!c->sp.fixed) HIPCHK(launch_cfl(c, c->mem, slot, 1, st));
   c->live[slot] = 1;
@@ -3499,7 +3508,7 @@ extern "C" amx_status amx_fv_pool_load(amx_fv_pool* p, int slot, const float* st
 extern "C" amx_status amx_fv_pool_release(amx_fv_pool* p, int slot, void* stream) {
   amx_status s = pool_slot_check(p, slot);
   if (s != AMX_OK) return s;
-  hipLaunchKernelGGL(k_slot_reset, dim3(1), dim3(1), 0, (hipStream_t)stream, p->c->ctl, slot, 0.f, 0);
+  hipLaunchKernelGGL(k_slot_reset, dim3(1), dim3(1), 0, (hipStream_t)stream, p->c->ctl, slot, 0.f, 0, 1);
   HIPCHK(hipGetLastError());
   p->c->live[slot] = 0;
   p->c->fresh[slot] = 0;
