@@ -392,6 +392,10 @@ __device__ __forceinline__ float ld32(const float* base, unsigned c) {
   return *(const float*)((const char*)base + (c << 2));
 }
 __device__ __forceinline__ void st32(float* base, unsigned c, float v) { *(float*)((char*)base + (c << 2)) = v; }
+// a store whose data this kernel does not read back (streaming: no reuse in the L2 expected)
+__device__ __forceinline__ void st32nt(float* base, unsigned c, float v) {
+  __builtin_nontemporal_store(v, (float*)((char*)base + (c << 2)));
+}
 
 // s_waitcnt immediate (gfx9 encoding): vmcnt 0, expcnt and lgkmcnt at their
 // maxima (no wait on them)
@@ -1502,8 +1506,8 @@ __global__ __launch_bounds__(((TA + 4) * (TBM + 4) + 255) / 256 * 256) void k_ma
           if (store1) {
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
-              st32(m.Bo[0][i], cell, B1[i]);
-              st32(m.vo[0][i], cell, v1[i]);
+              st32nt(m.Bo[0][i], cell, B1[i]);
+              st32nt(m.vo[0][i], cell, v1[i]);
             }
           }
           const float rr = mag_resid(rc.Bk, rc.vk, B1, v1);
@@ -1538,8 +1542,8 @@ __global__ __launch_bounds__(((TA + 4) * (TBM + 4) + 255) / 256 * 256) void k_ma
         const unsigned cell = (unsigned)(x - 1) * sx + col;
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
-          st32(m.Bo[1][i], cell, B2[i]);
-          st32(m.vo[1][i], cell, v2[i]);
+          st32nt(m.Bo[1][i], cell, B2[i]);
+          st32nt(m.vo[1][i], cell, v2[i]);
         }
         const float rr = mag_resid(Bp, vp, B2, v2);
         res1 = (rr != rr || rr > res1) ? rr : res1;
@@ -2463,8 +2467,7 @@ hipError_t launch_mag_pair(amx_fv_ctx* c, const Bufs& b, int L, int nspan, hipSt
 // Pairs (bit L / 2) of the speculative batch L = 0, 2, ... <= upto whose
 // first iterate is stored only by the pair starting at the slot's previous
 // final launch index (ctl.kprev, decided on the device).
-unsigned skip_mask_for(const amx_fv_ctx* c, int upto) {
-  (void)c;
+unsigned skip_mask_for(int upto) {
   unsigned m = 0u;
   for (int L = 0; L <= upto && L <= 62; L += 2) m |= 1u << (L >> 1);
   return m;
@@ -2526,7 +2529,7 @@ amx_status step_phase1(amx_fv_ctx* c, const Bufs& b, int nspan, hipStream_t st, 
   // pair reads the second): each slot's pairs store it only where its
   // previous step ended (ctl.kprev), and step_phase3 regenerates it in the
   // steps where it is final after all (skip_mask: the speculative pairs)
-  c->skip_mask = skip_mask_for(c, upto);
+  c->skip_mask = skip_mask_for(upto);
   for (int L = 0; L <= upto; L += 2) {
     HIPCHK(launch_mag_pair(c, b, L, nspan, st, L > 62 || !((c->skip_mask >> (L >> 1)) & 1u)));
     *klast = L + 1;
@@ -3558,11 +3561,11 @@ extern "C" amx_status amx_fv_pool_step_launch(amx_fv_pool* p, void* stream) {
     int upto = c->last_k < 1 ? 1 : c->last_k;
     if (upto > maxit) upto = maxit;
     klast = (upto / 2) * 2 + 1;
-    c->skip_mask = skip_mask_for(c, upto);  // graph replays do not run step_phase1 on the host
+    c->skip_mask = skip_mask_for(upto);  // graph replays do not run step_phase1 on the host
   } else {
     c->skip_mask = 0u;
   }
-  const int lk = c->cfg.mhd ? c->last_k : 0;  // fixes the launched pairs and which store their first iterate
+  const int lk = c->cfg.mhd ? c->last_k : 0;  // fixes the speculative pairs launched
   if (c->batch > 1) {
     // c->batch whole steps queued at once, each with its second half behind
     // its speculative launches; the device decides per slot whether they
