@@ -847,8 +847,8 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
         prim<DIM>(U, W1, ph);
       }
 #pragma unroll
-      for (int v = 0; v < NV; ++v) st32(outp[v], cell, W1[v]);
-      if (SRC == SRC_MAGFIX && a.pfix) st32(a.pfix + so, cell, q0[P]);
+      for (int v = 0; v < NV; ++v) st32nt(outp[v], cell, W1[v]);
+      if (SRC == SRC_MAGFIX && a.pfix) st32nt(a.pfix + so, cell, q0[P]);
       if (a.want_cfl) {
         const float cs = fsqrt(ph.gamma * W1[P] * frcp(W1[0]));
 #pragma unroll
