@@ -407,14 +407,19 @@ struct Raw {
   float f[N];
 };
 
-template <int DIM, int MODE>
+// NT: fields this kernel reads at one cell only (non-temporal loads)
+template <int DIM, int MODE, bool NT = false>
 __device__ __forceinline__ void load_raw(const float* const* s, unsigned c, Raw<DIM, MODE>& r) {
+  auto ld = [&](const float* b) {
+    const float* p = (const float*)((const char*)b + (c << 2));
+    return NT ? __builtin_nontemporal_load(p) : *p;
+  };
   if (MODE != SRC_MAGFIX) {
 #pragma unroll
-    for (int v = 0; v < DIM + 2; ++v) r.f[v] = ld32(s[v], c);
+    for (int v = 0; v < DIM + 2; ++v) r.f[v] = ld(s[v]);
   } else {
 #pragma unroll
-    for (int v = 0; v < 8; ++v) r.f[v] = (DIM == 2 && v == 4) ? 0.f : ld32(s[v], c);
+    for (int v = 0; v < 8; ++v) r.f[v] = (DIM == 2 && v == 4) ? 0.f : ld(s[v]);
   }
 }
 
@@ -695,7 +700,7 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
   // the 8-field SRC_MAGFIX centre would not be prefetched (register pressure):
   // the Heun stage after the magnetic sub-step takes the 5-field SRC_PFIX form
   constexpr bool PRE0 = HEUN && W0M != SRC_MAGFIX;
-  if (PRE0 && active) load_raw<DIM, W0M>(w0p, (DIM == 3 ? (unsigned)x0 * sx : 0u) + col, ra);
+  if (PRE0 && active) load_raw<DIM, W0M, true>(w0p, (DIM == 3 ? (unsigned)x0 * sx : 0u) + col, ra);
 
   float mx[3] = {0.f, 0.f, 0.f};
   // plane x: q0..q2 = planes x..x+2 (q2 loaded, or derived here from rq),
@@ -738,7 +743,7 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
         }
       }
       if (h >= 0) load_raw<DIM, SRC>(src, nplane + hoff, rh1);
-      if (PRE0 && active) load_raw<DIM, W0M>(w0p, nplane + col, r01);
+      if (PRE0 && active) load_raw<DIM, W0M, true>(w0p, nplane + col, r01);
     }
     if (owner) {
       if (DIM == 3) {
