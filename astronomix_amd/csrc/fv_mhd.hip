@@ -1291,6 +1291,7 @@ __global__ __launch_bounds__(((TA + 4) * (TBM + 4) + 255) / 256 * 256) void k_ma
   Ctl* const C = a.ctl + slot;
   if (!C->run) return;
   int L = a.L;
+  bool one = a.regen;  // the launch computes its first iterate only
   if (a.regen) {
     // Regeneration: the fixed point's final launch is read from the residuals
     // of the last executed pair; if it is that pair's first launch and the
@@ -1322,7 +1323,7 @@ __global__ __launch_bounds__(((TA + 4) * (TBM + 4) + 255) / 256 * 256) void k_ma
     C->resp[(L + 2) % NRES][tid] = 0u;
     C->resp[(L + 3) % NRES][tid] = 0u;
   }
-  const bool store1 = a.write_first || a.regen || L == C->kprev;
+  const bool store1 = a.write_first || one || L == C->kprev;
   const long so = slot * a.sstride;
   MagArgs m = a;  // slot-local pointers
   // B0 is in the state's rows or, in a loop that keeps B in the iterate
@@ -1438,7 +1439,7 @@ __global__ __launch_bounds__(((TA + 4) * (TBM + 4) + 255) / 256 * 256) void k_ma
       if (BCX && own) ghost_copies(sK1[0], e1);
     }
     __syncthreads();
-    if (own && !a.regen) {
+    if (own && !one) {
       float cE[3], cB[3], e1[6], B2[3], v2[3];
 #pragma unroll
       for (int i = 0; i < 6; ++i) {
@@ -1500,7 +1501,8 @@ __global__ __launch_bounds__(((TA + 4) * (TBM + 4) + 255) / 256 * 256) void k_ma
       }
       __syncthreads();
       float B1[3], v1[3], e1[6];
-      if (inner) {
+      // a launch of the first iterate only needs it at the own cells
+      if (one ? own : inner) {
         float cE[3], cB[3];
 #pragma unroll
         for (int i = 0; i < 6; ++i) SK[i] = sK[bx][i];
@@ -1519,14 +1521,19 @@ __global__ __launch_bounds__(((TA + 4) * (TBM + 4) + 255) / 256 * 256) void k_ma
           res0 = (rr != rr || rr > res0) ? rr : res0;
         }
         mag_derive3(B1, v1, rc.B0, rc.v0, a.euler, e1);
-        if (!gh) {
+        if (!one) {
+          if (!gh) {
 #pragma unroll
-          for (int i = 0; i < 6; ++i) sK1[bx][i][o1] = e1[i];
+            for (int i = 0; i < 6; ++i) sK1[bx][i][o1] = e1[i];
+          }
+          if (BCX && own) ghost_copies(sK1[bx], e1);
         }
-        if (BCX && own) ghost_copies(sK1[bx], e1);
       }
-      __syncthreads();
-      if (own && !a.regen && x - 1 >= x0 && x - 1 < x1) {
+      // No barrier here: the second iterate lags one plane and reads the
+      // first iterates of plane x - 1 (sK1[bx ^ 1]), written before this
+      // plane's barrier; sK1[bx] is next read at plane x + 1 and next
+      // written at plane x + 2, each behind a barrier of a later plane.
+      if (own && !one && x - 1 >= x0 && x - 1 < x1) {
         float cE[3], cB[3], B2[3], v2[3];
 #pragma unroll
         for (int i = 0; i < 6; ++i) SK1[i] = sK1[bx ^ 1][i];
@@ -1760,23 +1767,29 @@ __global__ void k_step_end_begin(Ctl* ctl, int keep_b, StepParams p) {
 }
 
 // B of slot 0 from its iterate set into the state's B rows (a loop that kept
-// B in the sets), then k_bset_clear hands B0 back to the state's rows.
+// B in the sets), then k_bset_clear hands B0 back to the state's rows. A copy
+// queued behind the last batch of a loop before the host has read the
+// control block (gate) does nothing when a step of the batch stalled: the
+// host copies after finishing that step.
 struct BsetArgs {
   const float* PB[4][3];  // B rows of the iterate sets
   float* dst[3];
   long n;
+  int gate;
 };
 
 __global__ void k_copy_bset(const Ctl* ctl, BsetArgs a) {
   const int bs = ctl->bset;
-  if (bs < 0) return;
+  if (bs < 0 || (a.gate && ctl->stall)) return;
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += (long)gridDim.x * blockDim.x) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) a.dst[k][i] = a.PB[bs][k][i];
   }
 }
 
-__global__ void k_bset_clear(Ctl* ctl) { ctl->bset = -1; }
+__global__ void k_bset_clear(Ctl* ctl, int gate) {
+  if (!(gate && ctl->stall)) ctl->bset = -1;
+}
 
 // The stalled steps of a queued batch (one slot per block) take part in
 // their further launches again, and only they: the steps queued behind a
@@ -2920,6 +2933,24 @@ amx_diag_desc fv_diag_desc(const amx_fv_ctx* c, double box_size) {
   return d;
 }
 
+// The final B of slot 0 from its iterate set into the state's B rows; gate:
+// nothing when a step of the queued batch stalled.
+hipError_t copy_b_home(amx_fv_ctx* c, const Bufs& bl, hipStream_t st, int gate) {
+  BsetArgs ba;
+  for (int j = 0; j < 4; ++j)
+    for (int k = 0; k < 3; ++k) ba.PB[j][k] = bl.P[j][k];
+  for (int k = 0; k < 3; ++k) ba.dst[k] = bl.B[k];
+  ba.n = c->ncell;
+  ba.gate = gate;
+  long blocks = (c->ncell + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(k_copy_bset, dim3(blocks), dim3(256), 0, st, c->ctl, ba);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_bset_clear, dim3(1), dim3(1), 0, st, c->ctl, gate);
+  return hipGetLastError();
+}
+
 // The step loop of _time_integration (time_integration.py:629-801); with a
 // snapshot driver, rows are taken as in time_integration.py:539-610, 736-745.
 amx_status fv_loop(amx_fv_ctx* c, float* state, const amx_fv_run* run, snap::Driver* snp, amx_fv_stats* stats,
@@ -2961,6 +2992,7 @@ amx_status fv_loop(amx_fv_ctx* c, float* state, const amx_fv_run* run, snap::Dri
   // the step-by-step path, which sorts out early-exited launches per step.
   const bool mbatch = c->cfg.mhd && c->keep_b && c->timing == 0;
   constexpr long BATCH = 16;
+  bool b_home = false;  // the final B is in the state's rows and hctl is current
   for (;;) {
     if (run->fixed_timestep) {
       if (steps >= run->num_timesteps) break;
@@ -2992,9 +3024,12 @@ amx_status fv_loop(amx_fv_ctx* c, float* state, const amx_fv_run* run, snap::Dri
       continue;
     }
     if (mbatch) {
-      long n = BATCH;
-      if (run->max_steps >= 0 && run->max_steps - steps < n) n = run->max_steps - steps;
-      if (run->fixed_timestep && run->num_timesteps - steps < n) n = run->num_timesteps - steps;
+      // steps left when the loop has a step bound; a tail of up to half a
+      // batch joins the batch before it (one readback fewer)
+      long left = -1;
+      if (run->max_steps >= 0) left = run->max_steps - steps;
+      if (run->fixed_timestep && (left < 0 || run->num_timesteps - steps < left)) left = run->num_timesteps - steps;
+      const long n = left >= 0 && left <= BATCH + BATCH / 2 ? left : BATCH;
       int klast = 0;
       StepParams spi = sp;
       spi.batch_first = 1;
@@ -3013,8 +3048,12 @@ amx_status fv_loop(amx_fv_ctx* c, float* state, const amx_fv_run* run, snap::Dri
           hipLaunchKernelGGL(k_step_end, dim3(1), dim3(1), 0, st, c->ctl, c->keep_b);
         HIPCHK(hipGetLastError());
       }
+      // the loop's last batch: B goes home behind it, unless a step stalled
+      const bool last = n == left;
+      if (last) HIPCHK(copy_b_home(c, bl, st, 1));
       HIPCHK(hipMemcpyAsync(c->hctl, c->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
       HIPCHK(hipStreamSynchronize(st));
+      b_home = last && !c->hctl[0].stall;
       if (c->hctl[0].stall) {
         // the stalled step's launches up to klast have run (every launch
         // queued behind it exited at once): the further pairs, then its
@@ -3074,21 +3113,13 @@ amx_status fv_loop(amx_fv_ctx* c, float* state, const amx_fv_run* run, snap::Dri
       if (s != AMX_OK) return s;
     }
   }
-  if (c->keep_b) {
-    BsetArgs ba;
-    for (int j = 0; j < 4; ++j)
-      for (int k = 0; k < 3; ++k) ba.PB[j][k] = bl.P[j][k];
-    for (int k = 0; k < 3; ++k) ba.dst[k] = bl.B[k];
-    ba.n = c->ncell;
-    long blocks = (c->ncell + 255) / 256;
-    if (blocks > 8192) blocks = 8192;
-    hipLaunchKernelGGL(k_copy_bset, dim3(blocks), dim3(256), 0, st, c->ctl, ba);
-    HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_bset_clear, dim3(1), dim3(1), 0, st, c->ctl);
-    HIPCHK(hipGetLastError());
+  if (c->keep_b && !b_home) {
+    HIPCHK(copy_b_home(c, bl, st, 0));
   }
-  HIPCHK(hipMemcpyAsync(c->hctl, c->ctl, offsetof(Ctl, cflp), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
+  if (!b_home) {
+    HIPCHK(hipMemcpyAsync(c->hctl, c->ctl, offsetof(Ctl, cflp), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
   time_resolve(c);
   if (snp) snp->finish(steps);
   if (stats) {
@@ -3538,6 +3569,7 @@ extern "C" amx_status amx_fv_pool_read(amx_fv_pool* p, int slot, float* dst, voi
       for (int k = 0; k < 3; ++k) ba.PB[j][k] = b.P[j][k] + so;
     for (int k = 0; k < 3; ++k) ba.dst[k] = c->mem + so + (long)(c->NV + k) * c->ncell;
     ba.n = c->ncell;
+    ba.gate = 0;
     long blocks = (c->ncell + 255) / 256;
     if (blocks > 8192) blocks = 8192;
     hipLaunchKernelGGL(k_copy_bset, dim3(blocks), dim3(256), 0, st, c->ctl + slot, ba);
