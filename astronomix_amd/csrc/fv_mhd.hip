@@ -517,12 +517,17 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
   constexpr int NFA = (TA + 1) * TB, NF = NFA + TA * (TB + 1);
   static_assert(NFA == NT && NF - NFA <= NT, "flux rounds assume (TA+1)*TB threads");
   static_assert(NH <= NT, "every halo cell of a plane needs its own thread (TA >= 4)");
-  __shared__ float sW[NV][(TA + 4) * WW];
-  __shared__ float sFA[NV][(TA + 1) * TB];
-  __shared__ float sFB[NV][TA * (TB + 1)];
+  // LAG (3D, limiters without plane factors): the cell update runs one
+  // plane behind the interface fluxes, and with the plane arrays double
+  // buffered one barrier per plane orders both phases
+  constexpr bool LAG = DIM == 3 && !LIMX;
+  constexpr int NBUF = LAG ? 2 : 1;
+  __shared__ float sW[NBUF][NV][(TA + 4) * WW];
+  __shared__ float sFA[NBUF][NV][(TA + 1) * TB];
+  __shared__ float sFB[NBUF][NV][TA * (TB + 1)];
   constexpr bool AMS = RS == AMX_HYBRID_HLLC || RS == AMX_AM_HLLC;
   static_assert(!(AMS && BCX), "the compression switch of ghost cells is not defined here");
-  __shared__ unsigned char sG[AMS ? (TA + 4) * WW : 1];
+  __shared__ unsigned char sG[NBUF][AMS ? (TA + 4) * WW : 1];
   // VAN_ALBADA_PP factors of the plane cells: alpha_A, kappa_A, alpha_B, kappa_B, beta
   __shared__ float sPF[LIMX ? 5 : 1][LIMX ? (TA + 4) * WW : 1];
 
@@ -700,19 +705,72 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
   // the 8-field SRC_MAGFIX centre would not be prefetched (register pressure):
   // the Heun stage after the magnetic sub-step takes the 5-field SRC_PFIX form
   constexpr bool PRE0 = HEUN && W0M != SRC_MAGFIX;
-  if (PRE0 && active) load_raw<DIM, W0M, true>(w0p, (DIM == 3 ? (unsigned)x0 * sx : 0u) + col, ra);
+  // (LAG: plane x's centre is loaded in plane x, for its update in plane x + 1)
+  if (!LAG && PRE0 && active) load_raw<DIM, W0M, true>(w0p, (DIM == 3 ? (unsigned)x0 * sx : 0u) + col, ra);
 
   float mx[3] = {0.f, 0.f, 0.f};
+  // Update of the thread's cell in plane xu from its primitive state q, the
+  // x-fluxes through its lower (Fl) and upper (Fr) faces, the plane fluxes
+  // FAu / FBu and (Heun) the prefetched centre r0.
+  auto update = [&](const int xu, const float (&q)[NV], const float (&Fl)[NV], const float (&Fr)[NV],
+                    const Raw<DIM, W0M>& r0, const float (*FAu)[(TA + 1) * TB], const float (*FBu)[TA * (TB + 1)]) {
+    const unsigned cell = (DIM == 3 ? (unsigned)xu * sx : 0u) + col;
+    float U[NV], W1[NV];
+    cons<DIM>(q, U, ph);
+    if (DIM == 3) {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) U[v] = U[v] + (Fl[v] - Fr[v]) * coef;
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) U[v] = U[v] + (FAu[v][ta * TB + tb] - FAu[v][(ta + 1) * TB + tb]) * coef;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) U[v] = U[v] + (FBu[v][ta * (TB + 1) + tb] - FBu[v][ta * (TB + 1) + tb + 1]) * coef;
+    prim<DIM>(U, W1, ph);
+    if (HEUN) {
+      float W0[NV], U0[NV], U2[NV];
+      if (PRE0) {
+        derive_w<DIM, W0M>(r0, W0);
+      } else {
+        load_w<DIM, W0M>(w0p, cell, W0);
+      }
+      cons<DIM>(W0, U0, ph);
+      cons<DIM>(W1, U2, ph);
+#pragma unroll
+      for (int v = 0; v < NV; ++v) U[v] = 0.5f * (U0[v] + U2[v]);
+      prim<DIM>(U, W1, ph);
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) st32nt(outp[v], cell, W1[v]);
+    if (SRC == SRC_MAGFIX && a.pfix) st32nt(a.pfix + so, cell, q[P]);
+    if (a.want_cfl) {
+      const float cs = fsqrt(ph.gamma * W1[P] * frcp(W1[0]));
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) {
+        const float m = fabsf(W1[1 + k]) + cs;
+        mx[k] = (m != m || m > mx[k]) ? m : mx[k];
+      }
+    }
+    if (a.copy_b) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) st32(a.bdst[k] + so, cell, ld32(a.selb[par][k] + so, cell));
+    }
+  };
   // plane x: q0..q2 = planes x..x+2 (q2 loaded, or derived here from rq),
   // q3 receives plane x+3; d0 / Fp: slope and x-flux of plane x (from the
   // previous plane), d1 / Fn: those of plane x+1; rh0 / r00: halo and Heun
-  // centre of plane x, rh1 / r01 receive plane x+1
+  // centre of plane x, rh1 / r01 receive plane x+1 (LAG: r00 receives plane
+  // x, r01 holds plane x-1, whose cell update runs in this plane)
   auto plane_step = [&](const int x, float (&q0)[NV], float (&q1)[NV], float (&q2)[NV], float (&q3)[NV],
                         float (&d0)[NV], float (&d1)[NV], float (&Fp)[NV], float (&Fn)[NV], Raw<DIM, SRC>& rh0,
                         Raw<DIM, SRC>& rh1, Raw<DIM, W0M>& r00, Raw<DIM, W0M>& r01) {
     const unsigned plane = DIM == 3 ? (unsigned)x * sx : 0u;
     const unsigned nplane = DIM == 3 ? (unsigned)(x + 1) * sx : 0u;
     const bool more = x + 1 < x1;
+    const int bb = LAG ? (x & 1) : 0;  // plane buffers of plane x
+    float(*const W)[(TA + 4) * WW] = sW[bb];
+    float(*const FA)[(TA + 1) * TB] = sFA[bb];
+    float(*const FB)[TA * (TB + 1)] = sFB[bb];
+    unsigned char* const G = sG[bb];
     // The loads issued one plane ago (q2, rh0, r00) have had the previous
     // plane's flux phase to land: wait for them here, before this plane's
     // prefetch is issued. Without it, a use of q2 / rh0 / r00 behind the new
@@ -743,10 +801,12 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
         }
       }
       if (h >= 0) load_raw<DIM, SRC>(src, nplane + hoff, rh1);
-      if (PRE0 && active) load_raw<DIM, W0M, true>(w0p, nplane + col, r01);
+      if (!LAG && PRE0 && active) load_raw<DIM, W0M, true>(w0p, nplane + col, r01);
     }
-    if (owner) {
-      if (DIM == 3) {
+    if (LAG && PRE0 && active) load_raw<DIM, W0M, true>(w0p, plane + col, r00);
+    // x-flux through the upper face of the plane's cell (face x + 1/2)
+    auto xflux = [&]() {
+      if (owner && DIM == 3) {
         float wl[NV], wr[NV];
         float fa1 = 1.f, fk1 = 1.f, fb1 = 1.f;
         if (pp) ppx(x + 1, fa1, fk1, fb1);
@@ -760,14 +820,17 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
         riemann<DIM, 1, RS>(wl, wr, Fn, ph, alf[0],
                             AMS && gsw[(unsigned)wrapi(x + 1, a.nx) * sx + col]);
       }
+    };
+    if (!LAG) xflux();
+    if (owner) {
 #pragma unroll
-      for (int v = 0; v < NV; ++v) sW[v][(ta + 2) * WW + tb + 2] = q0[v];
+      for (int v = 0; v < NV; ++v) W[v][(ta + 2) * WW + tb + 2] = q0[v];
       if (BCX) {
         // a ghost column's cells enter the plane stencils with their flips
-        if (fa) sW[AXA][(ta + 2) * WW + tb + 2] = -q0[AXA];
-        if (fb) sW[AXB][(ta + 2) * WW + tb + 2] = -q0[AXB];
+        if (fa) W[AXA][(ta + 2) * WW + tb + 2] = -q0[AXA];
+        if (fb) W[AXB][(ta + 2) * WW + tb + 2] = -q0[AXB];
       }
-      if (AMS) sG[(ta + 2) * WW + tb + 2] = gsw[plane + col];
+      if (AMS) G[(ta + 2) * WW + tb + 2] = gsw[plane + col];
       if (pp) {
 #pragma unroll
         for (int i = 0; i < 5; ++i) sPF[i][(ta + 2) * WW + tb + 2] = ld32(pfp[i], plane + col);
@@ -775,8 +838,8 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
     }
     if (h >= 0) {
 #pragma unroll
-      for (int v = 0; v < NV; ++v) sW[v][(hr + 2) * WW + hc + 2] = hw[v];
-      if (AMS) sG[(hr + 2) * WW + hc + 2] = gsw[plane + hoff];
+      for (int v = 0; v < NV; ++v) W[v][(hr + 2) * WW + hc + 2] = hw[v];
+      if (AMS) G[(hr + 2) * WW + hc + 2] = gsw[plane + hoff];
       if (pp) {
 #pragma unroll
         for (int i = 0; i < 5; ++i) sPF[i][(hr + 2) * WW + hc + 2] = ld32(pfp[i], plane + hoff);
@@ -792,7 +855,7 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
       const int o = (r + 2) * WW + c + 2;
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
-        const float w0 = sW[v][o - 2 * WW], w1 = sW[v][o - WW], w2 = sW[v][o], w3 = sW[v][o + WW];
+        const float w0 = W[v][o - 2 * WW], w1 = W[v][o - WW], w2 = W[v][o], w3 = W[v][o + WW];
         float s1 = slope<LIMX>(w0, w1, w2, lim, ldx), s2 = slope<LIMX>(w1, w2, w3, lim, ldx);
         if (pp) {
           s1 = s1 * fsel(sPF[0][o - WW], sPF[1][o - WW], sPF[4][o - WW], v);
@@ -801,9 +864,9 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
         wl[v] = w1 + s1;
         wr[v] = w2 - s2;
       }
-      riemann<DIM, AXA, RS>(wl, wr, f, ph, alf[AXA - 1], AMS && sG[o]);
+      riemann<DIM, AXA, RS>(wl, wr, f, ph, alf[AXA - 1], AMS && G[o]);
 #pragma unroll
-      for (int v = 0; v < NV; ++v) sFA[v][tid] = f[v];
+      for (int v = 0; v < NV; ++v) FA[v][tid] = f[v];
     }
     if (tid < NF - NFA) {
       float wl[NV], wr[NV], f[NV];
@@ -811,7 +874,7 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
       const int o = (r + 2) * WW + c + 2;
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
-        const float w0 = sW[v][o - 2], w1 = sW[v][o - 1], w2 = sW[v][o], w3 = sW[v][o + 1];
+        const float w0 = W[v][o - 2], w1 = W[v][o - 1], w2 = W[v][o], w3 = W[v][o + 1];
         float s1 = slope<LIMX>(w0, w1, w2, lim, ldx), s2 = slope<LIMX>(w1, w2, w3, lim, ldx);
         if (pp) {
           s1 = s1 * fsel(sPF[2][o - 1], sPF[3][o - 1], sPF[4][o - 1], v);
@@ -820,53 +883,25 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
         wl[v] = w1 + s1;
         wr[v] = w2 - s2;
       }
-      riemann<DIM, AXB, RS>(wl, wr, f, ph, alf[AXB - 1], AMS && sG[o]);
+      riemann<DIM, AXB, RS>(wl, wr, f, ph, alf[AXB - 1], AMS && G[o]);
 #pragma unroll
-      for (int v = 0; v < NV; ++v) sFB[v][tid] = f[v];
+      for (int v = 0; v < NV; ++v) FB[v][tid] = f[v];
+    }
+    if constexpr (LAG) {
+      // the cell of plane x - 1: its plane fluxes were written before this
+      // plane's barrier; its state is the thread's own entry of that plane's
+      // buffer, and its faces are x - 3/2 (still in Fn) and x - 1/2 (Fp)
+      if (x > x0 && active) {
+        float qm[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) qm[v] = sW[bb ^ 1][v][(ta + 2) * WW + tb + 2];
+        update(x - 1, qm, Fn, Fp, r01, sFA[bb ^ 1], sFB[bb ^ 1]);
+      }
+      xflux();
+      return;
     }
     __syncthreads();
-    if (active) {
-      const unsigned cell = plane + col;
-      float U[NV], W1[NV];
-      cons<DIM>(q0, U, ph);
-      if (DIM == 3) {
-#pragma unroll
-        for (int v = 0; v < NV; ++v) U[v] = U[v] + (Fp[v] - Fn[v]) * coef;
-      }
-#pragma unroll
-      for (int v = 0; v < NV; ++v) U[v] = U[v] + (sFA[v][ta * TB + tb] - sFA[v][(ta + 1) * TB + tb]) * coef;
-#pragma unroll
-      for (int v = 0; v < NV; ++v) U[v] = U[v] + (sFB[v][ta * (TB + 1) + tb] - sFB[v][ta * (TB + 1) + tb + 1]) * coef;
-      prim<DIM>(U, W1, ph);
-      if (HEUN) {
-        float W0[NV], U0[NV], U2[NV];
-        if (PRE0) {
-          derive_w<DIM, W0M>(r00, W0);
-        } else {
-          load_w<DIM, W0M>(w0p, cell, W0);
-        }
-        cons<DIM>(W0, U0, ph);
-        cons<DIM>(W1, U2, ph);
-#pragma unroll
-        for (int v = 0; v < NV; ++v) U[v] = 0.5f * (U0[v] + U2[v]);
-        prim<DIM>(U, W1, ph);
-      }
-#pragma unroll
-      for (int v = 0; v < NV; ++v) st32nt(outp[v], cell, W1[v]);
-      if (SRC == SRC_MAGFIX && a.pfix) st32nt(a.pfix + so, cell, q0[P]);
-      if (a.want_cfl) {
-        const float cs = fsqrt(ph.gamma * W1[P] * frcp(W1[0]));
-#pragma unroll
-        for (int k = 0; k < DIM; ++k) {
-          const float m = fabsf(W1[1 + k]) + cs;
-          mx[k] = (m != m || m > mx[k]) ? m : mx[k];
-        }
-      }
-      if (a.copy_b) {
-#pragma unroll
-        for (int k = 0; k < 3; ++k) st32(a.bdst[k] + so, cell, ld32(a.selb[par][k] + so, cell));
-      }
-    }
+    if (active) update(x, q0, Fp, Fn, r00, FA, FB);
   };
   if (DIM == 3) {
     for (int x = x0; x < x1; x += 4) {
@@ -874,6 +909,20 @@ __global__ __launch_bounds__((TA + 1) * TB) __attribute__((amdgpu_waves_per_eu(A
       if (x + 1 < x1) plane_step(x + 1, qb, qc, qd, qa, db, da, Fb, Fa, hb, ha, rb, ra);
       if (x + 2 < x1) plane_step(x + 2, qc, qd, qa, qb, da, db, Fa, Fb, ha, hb, ra, rb);
       if (x + 3 < x1) plane_step(x + 3, qd, qa, qb, qc, db, da, Fb, Fa, hb, ha, rb, ra);
+    }
+    if constexpr (LAG) {
+      // the update of the last plane
+      __syncthreads();
+      if (active) {
+        const int xl = x1 - 1, bl = xl & 1;
+        float ql[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) ql[v] = sW[bl][v][(ta + 2) * WW + tb + 2];
+        if (((xl - x0) & 1) == 0)
+          update(xl, ql, Fa, Fb, ra, sFA[bl], sFB[bl]);
+        else
+          update(xl, ql, Fb, Fa, rb, sFA[bl], sFB[bl]);
+      }
     }
   } else {
     plane_step(0, qa, qb, qc, qd, da, db, Fa, Fb, ha, hb, ra, rb);
