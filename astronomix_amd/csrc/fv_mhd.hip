@@ -89,6 +89,8 @@ struct Ctl {
   int held;                     // the fixed point had not settled when k_mag_final ran: gated stage kernels return
   int kprev;                    // final launch index of the slot's previous step (k_mag_final): the speculative
                                 // pair starting there stores its first iterate, the others leave it out
+  int regen_ev;                 // steps of the current batch that ended on a first iterate their pair did not
+                                // store (previous final index known): the next batch queues regeneration launches
 };
 
 // Iterate set of fixed-point launch k: the three sets other than bset
@@ -1653,14 +1655,22 @@ __global__ __launch_bounds__(((TA + 4) * (TBM + 4) + 255) / 256 * 256) void k_ma
 // is queued before the host has seen the residuals; if the reference's loop
 // would still run, it sets held (the gated stage kernels behind it return)
 // and leaves iters for the call after the further launches.
-__global__ void k_mag_final(Ctl* ctl, float tol, int max_iter, int spec) {
+//
+// A final first iterate that its pair did not store (skip_mask: the
+// speculative pairs, which store it only at kprev) is recomputed by the
+// regeneration launch queued before this one; a speculative call without
+// one queued (regen_queued = 0) holds the step instead, and the host
+// queues the regeneration when it finishes the step.
+__global__ void k_mag_final(Ctl* ctl, float tol, int max_iter, int spec, unsigned skip_mask, int regen_queued) {
   Ctl* const c = ctl + blockIdx.x;
   if (!c->run) return;
   const int L = c->iters - 1;
   const float r0 = __uint_as_float(slots_max(c->resp[L % NRES]));
   const float r1 = __uint_as_float(slots_max(c->resp[(L + 1) % NRES]));
   const int f = pair_final(L, r0, r1, tol, max_iter);
-  if (f < 0 && spec) {
+  const bool unstored = f >= 0 && f == L && L <= 62 && ((skip_mask >> (L >> 1)) & 1u) && L != c->kprev;
+  if (spec && unstored && c->kprev >= 0) c->regen_ev += 1;
+  if ((f < 0 || (unstored && !regen_queued)) && spec) {
     c->held = 1;
     c->stall = 1;
     return;
@@ -1754,7 +1764,10 @@ __device__ __forceinline__ void zero_cflp(Ctl* c) {
 // One thread per slot (slot = blockIdx.x): the step's dt, and whether the
 // slot runs this step (live and not yet at t_end / num_timesteps).
 __device__ void step_begin(Ctl* c, const StepParams& p) {
-  if (p.batch_first) c->iters_bmax = 0;
+  if (p.batch_first) {
+    c->iters_bmax = 0;
+    c->regen_ev = 0;
+  }
   if (!c->live) {
     c->run = 0;
     return;
@@ -1877,6 +1890,7 @@ __global__ void k_slot_reset(Ctl* ctl, int slot0, float t_end, int live, int fre
   c->bset = -1;
   c->held = 0;
   c->stall = 0;
+  c->regen_ev = 0;
   if (fresh) c->kprev = -1;
   c->iters_sum = 0;
   c->iters_max = 0;
@@ -1960,6 +1974,7 @@ struct amx_fv_ctx {
   Phys* dph;        // cap entries
   Phys* hph;        // pinned host copies
   int last_k;
+  int regen_q;      // device loop: the next batch queues regeneration launches (a step of the last one needed it)
   StepParams sp;    // pool stepping
   int span;         // pool: slots [0, span) take part in the launched step
   int stepping;     // pool: a step is launched and not finished
@@ -2258,6 +2273,7 @@ void init_geometry(amx_fv_ctx* c, const amx_fv_config* cfg, int cap, int own_sta
   const long per_slot = c->ncell * (ws_fields(cfg, own_state) + (own_state ? c->NS : 0));
   c->sstride = own_state ? per_slot : 0;
   c->last_k = 4;
+  c->regen_q = 1;
 }
 
 // Geometry, launch shapes and allocations of a context of cap slots.
@@ -2656,13 +2672,17 @@ amx_status fixed_point_more(amx_fv_ctx* c, const Bufs& b, int nspan, hipStream_t
 
 // Phase 3: S_A(dt/2) on the magnetically updated gas (v <- v_K, energy fix on
 // the fly); the final iterate of every slot is picked on the device.
-amx_status step_phase3(amx_fv_ctx* c, const Bufs& b, int nspan, hipStream_t st, int want_cfl, int spec) {
+// regen: queue the regeneration launch (a speculative call without it holds
+// a step that ends on an unstored first iterate; see k_mag_final).
+amx_status step_phase3(amx_fv_ctx* c, const Bufs& b, int nspan, hipStream_t st, int want_cfl, int spec,
+                       int regen = 1) {
   if (!c->cfg.mhd) return AMX_OK;
   const int NV = c->NV;
   // the final iterate, where it is the first of a pair that did not store it
-  if (c->skip_mask) HIPCHK(launch_mag_pair(c, b, 0, nspan, st, 1, 1));
+  const int rq = c->skip_mask && (regen || !spec);
+  if (rq) HIPCHK(launch_mag_pair(c, b, 0, nspan, st, 1, 1));
   hipLaunchKernelGGL(k_mag_final, dim3(nspan), dim3(1), 0, st, c->ctl, c->cfg.fixed_point_tol,
-                     c->cfg.max_fixed_point_iterations, spec);
+                     c->cfg.max_fixed_point_iterations, spec, c->skip_mask, rq);
   HIPCHK(hipGetLastError());
   StageArgs b1 = stage_base(c, 0.5f);
   for (int v = 0; v < NV; ++v) {
@@ -2907,6 +2927,7 @@ __global__ void k_noalloc_begin(Ctl* c, Phys* ph, Phys v, const float* dtp) {
   c->run = 1;
   c->bset = -1;
   c->kprev = -1;
+  c->regen_ev = 0;
   for (int i = 0; i < 4; ++i) c->cfl[i] = 0u;
   zero_cflp(c);
   fixed_point_reset(c);
@@ -3088,7 +3109,7 @@ amx_status fv_loop(amx_fv_ctx* c, float* state, const amx_fv_run* run, snap::Dri
       for (long i = 0; i < n; ++i) {
         amx_status s = step_phase1(c, bl, 1, st, !run->fixed_timestep, &klast);
         if (s != AMX_OK) return s;
-        s = step_phase3(c, bl, 1, st, !run->fixed_timestep, 1);
+        s = step_phase3(c, bl, 1, st, !run->fixed_timestep, 1, c->regen_q);
         if (s != AMX_OK) return s;
         // this step's end and the next one's beginning
         if (i + 1 < n)
@@ -3103,6 +3124,9 @@ amx_status fv_loop(amx_fv_ctx* c, float* state, const amx_fv_run* run, snap::Dri
       HIPCHK(hipMemcpyAsync(c->hctl, c->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
       HIPCHK(hipStreamSynchronize(st));
       b_home = last && !c->hctl[0].stall;
+      // regeneration launches in the next batch only if a step of this one
+      // ended on an unstored first iterate (otherwise such a step stalls)
+      c->regen_q = c->hctl[0].regen_ev > 0;
       if (c->hctl[0].stall) {
         // the stalled step's launches up to klast have run (every launch
         // queued behind it exited at once): the further pairs, then its

@@ -353,3 +353,35 @@ def test_final_first_iterate_regenerated(gpu, oracle, shape):
             regenerated.append(k)
         prev = k
     assert 0 in regenerated and 2 in regenerated, regenerated
+
+
+def test_loop_without_regeneration_launches(gpu):
+    """The device loop queues regeneration launches only in batches after
+    one whose steps needed them; without them, a step that ends on a first
+    iterate its pair did not store is held and finished by the host (the
+    regeneration, then its second half). Loop calls of fixed dt on one
+    context drive the final launch index 3, 3, 3 | 2, 2, 2 | 3, 3 | 4, 4 |
+    2, 2: the first steps of the 2-calls take that route. The loops equal
+    the same steps one at a time (which always queue the regeneration) bit
+    for bit, with the same iteration counts."""
+    import torch
+    print("[gpu] loop without regeneration launches", flush=True)
+    state = W.random_smooth((16, 16, 16), True, seed=8)
+    cfg = W.fv_config(3, 16, mhd=True, riemann=HLL)
+    nc = native_config(state.shape, cfg, 5.0 / 3.0, 0.4)
+    loop = _native.FvContext(_native.AmxFvConfig.from_buffer_copy(nc))
+    single = _native.FvContext(_native.AmxFvConfig.from_buffer_copy(nc))
+    s_loop = torch.from_numpy(np.ascontiguousarray(state)).cuda()
+    s_step = s_loop.clone()
+    ks = []
+    for dt, n in [(1e-3, 3), (3e-4, 3), (1e-3, 2), (4e-3, 2), (3e-4, 2)]:
+        t_end = float(np.float32(dt) * np.float32(n))
+        run = _native.AmxFvRun(t_end=t_end, fixed_timestep=1, num_timesteps=n, exact_end_time=0, max_steps=-1)
+        st = loop.time_integration(s_loop, run)
+        d = torch.tensor(np.float32(t_end) / np.float32(n), dtype=torch.float32, device="cuda")
+        kk = [single.evolve_state(s_step, d) for _ in range(n)]
+        torch.cuda.synchronize()
+        assert st.steps == n and st.fixed_point_iterations == sum(kk), (dt, st.fixed_point_iterations, kk)
+        assert torch.equal(s_loop, s_step), (dt, kk)
+        ks += kk
+    assert ks == [3, 3, 3, 2, 2, 2, 3, 3, 4, 4, 2, 2], ks
