@@ -1693,7 +1693,9 @@ struct CflArgs {
   int slot0;
 };
 
-template <int DIM>
+// V4: the rows are 16-byte aligned and n is a multiple of 4; each thread
+// reads four cells of every row per load.
+template <int DIM, bool V4>
 __global__ void k_cfl(const CflArgs a) {
   const int slot = a.slot0 + blockIdx.y;
   Ctl* const C = a.ctl + slot;
@@ -1706,14 +1708,29 @@ __global__ void k_cfl(const CflArgs a) {
   const float* u1 = a.w[2] + so;
   const float* u2 = DIM == 3 ? a.w[3] + so : nullptr;
   float mx[3] = {0.f, 0.f, 0.f};
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += (long)gridDim.x * blockDim.x) {
-    const float cs = fsqrt(gamma * p[i] * frcp(rho[i]));
-    const float u[3] = {u0[i], u1[i], DIM == 3 ? u2[i] : 0.f};
+  auto cell = [&](float r, float pp, float x, float y, float z) {
+    const float cs = fsqrt(gamma * pp * frcp(r));
+    const float u[3] = {x, y, z};
 #pragma unroll
     for (int k = 0; k < DIM; ++k) {
       const float m = fabsf(u[k]) + cs;
       mx[k] = (m != m || m > mx[k]) ? m : mx[k];
     }
+  };
+  if (V4) {
+    const long n4 = a.n >> 2;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (long)gridDim.x * blockDim.x) {
+      const float4 r = ((const float4*)rho)[i], pp = ((const float4*)p)[i];
+      const float4 x = ((const float4*)u0)[i], y = ((const float4*)u1)[i];
+      const float4 z = DIM == 3 ? ((const float4*)u2)[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+      cell(r.x, pp.x, x.x, y.x, z.x);
+      cell(r.y, pp.y, x.y, y.y, z.y);
+      cell(r.z, pp.z, x.z, y.z, z.z);
+      cell(r.w, pp.w, x.w, y.w, z.w);
+    }
+  } else {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += (long)gridDim.x * blockDim.x)
+      cell(rho[i], p[i], u0[i], u1[i], DIM == 3 ? u2[i] : 0.f);
   }
   __shared__ unsigned sred[DIM][4];
 #pragma unroll
@@ -2513,12 +2530,21 @@ hipError_t launch_cfl(const amx_fv_ctx* c, const float* state, int slot0, int ns
   a.sstride = c->sstride;
   a.n = n;
   a.slot0 = slot0;
-  long blocks = (n + 255) / 256;
+  bool v4 = (n & 3) == 0 && (a.sstride & 3) == 0;
+  for (int v = 0; v <= c->cfg.dim; ++v) v4 = v4 && ((uintptr_t)a.w[v] & 15) == 0;
+  v4 = v4 && ((uintptr_t)a.p & 15) == 0;
+  long blocks = ((v4 ? n / 4 : n) + 255) / 256;
   if (blocks > 4096) blocks = 4096;
-  if (c->cfg.dim == 3)
-    hipLaunchKernelGGL(k_cfl<3>, dim3(blocks, nspan), dim3(256), 0, st, a);
-  else
-    hipLaunchKernelGGL(k_cfl<2>, dim3(blocks, nspan), dim3(256), 0, st, a);
+  if (c->cfg.dim == 3) {
+    if (v4)
+      hipLaunchKernelGGL((k_cfl<3, true>), dim3(blocks, nspan), dim3(256), 0, st, a);
+    else
+      hipLaunchKernelGGL((k_cfl<3, false>), dim3(blocks, nspan), dim3(256), 0, st, a);
+  } else if (v4) {
+    hipLaunchKernelGGL((k_cfl<2, true>), dim3(blocks, nspan), dim3(256), 0, st, a);
+  } else {
+    hipLaunchKernelGGL((k_cfl<2, false>), dim3(blocks, nspan), dim3(256), 0, st, a);
+  }
   return hipGetLastError();
 }
 
