@@ -52,6 +52,9 @@ CASES = [
     ("random_3d_24_hll", lambda: W.random_smooth((24, 24, 24), True, seed=1), 3, True, HLL, 0.4, 0.01),
     ("random_3d_20_hllc", lambda: W.random_smooth((20, 20, 20), True, seed=2), 3, True, HLLC, 0.4, 0.01),
     ("random_3d_72_hll", lambda: W.random_smooth((72, 72, 72), True, seed=3), 3, True, HLL, 0.4, 0.004),
+    # 21 planes: x chunks of 8, 8 and 5 (the stage kernels' lagged update
+    # ends a chunk of odd length on the other register and buffer parity)
+    ("random_3d_21_hllc", lambda: W.random_smooth((21, 21, 21), True, seed=6), 3, True, HLLC, 0.4, 0.01),
     ("hydro_3d_24_hll", lambda: W.random_smooth((24, 24, 24), False, seed=4), 3, False, HLL, 0.4, 0.02),
     ("ot_2d_64_hll", lambda: W.orszag_tang_2d(64), 2, True, HLL, 0.4, 0.1),
     ("ot_2d_100_hllc", lambda: W.orszag_tang_2d(100), 2, True, HLLC, 0.4, 0.05),
