@@ -1562,10 +1562,13 @@ __global__ __launch_bounds__(((TA + 4) * (TBM + 4) + 255) / 256 * 256) void k_ma
         if (own && x >= x0 && x < x1) {
           const unsigned cell = (unsigned)x * sx + col;
           if (store1) {
+            // plain stores: the pair that also keeps its first iterate (the
+            // previous step ended on it) was faster so than with
+            // non-temporal ones
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
-              st32nt(m.Bo[0][i], cell, B1[i]);
-              st32nt(m.vo[0][i], cell, v1[i]);
+              st32(m.Bo[0][i], cell, B1[i]);
+              st32(m.vo[0][i], cell, v1[i]);
             }
           }
           const float rr = mag_resid(rc.Bk, rc.vk, B1, v1);
