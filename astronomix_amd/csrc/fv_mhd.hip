@@ -1562,9 +1562,9 @@ __global__ __launch_bounds__(((TA + 4) * (TBM + 4) + 255) / 256 * 256) void k_ma
         if (own && x >= x0 && x < x1) {
           const unsigned cell = (unsigned)x * sx + col;
           if (store1) {
-            // plain stores: the pair that also keeps its first iterate (the
-            // previous step ended on it) was faster so than with
-            // non-temporal ones
+            // plain stores (with non-temporal ones the pair that keeps its
+            // first iterate too, where the previous step ended on it, was
+            // slower)
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
               st32(m.Bo[0][i], cell, B1[i]);
