@@ -1608,8 +1608,15 @@ __global__ __launch_bounds__(((TA + 4) * (TBM + 4) + 255) / 256 * 256) void k_ma
         const unsigned cell = (unsigned)(x - 1) * sx + col;
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
-          st32nt(m.Bo[1][i], cell, B2[i]);
-          st32nt(m.vo[1][i], cell, v2[i]);
+          // non-temporal stores for launch 0, plain ones for the later
+          // pairs (each the faster in A/B runs at 256^3)
+          if (FIRST) {
+            st32nt(m.Bo[1][i], cell, B2[i]);
+            st32nt(m.vo[1][i], cell, v2[i]);
+          } else {
+            st32(m.Bo[1][i], cell, B2[i]);
+            st32(m.vo[1][i], cell, v2[i]);
+          }
         }
         const float rr = mag_resid(Bp, vp, B2, v2);
         res1 = (rr != rr || rr > res1) ? rr : res1;
